@@ -1,0 +1,147 @@
+"""Benchmark: shaded primary rays per second on one MI355X.
+
+Workload (one step = one frame): the cheese scene — a cube of edge 2 at the
+origin minus a balanced Union of 512 spheres (centres U[-1,1]^3, radii
+U[0.05, 0.2], seed 512) — at 1920x1080, camera at (3, 2.5, 4) looking at the
+origin with a 60 degree vertical fov, light toward (1,1,1), Phong ka 0.1,
+kd 0.7, ks 0.2, n 32.  Every ray is traced through the CSG state machine,
+shaded, and writes RGBA (16 B), depth (4 B), primitive id (4 B) and flags
+(1 B).  Inputs are resident in HBM before timing starts.
+
+Prints one JSON line (rank 0).  --gpus above 1 is refused: this build runs on
+one GPU.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "cuda-csg-tree-raycasting_amd"))
+
+FLOP_FIXTURE = os.path.join(ROOT, "tests", "golden", "events_cheese512_1080p.json")
+TRAFFIC_FIXTURE = os.path.join(ROOT, "profiles", "traffic_render_cheese512_1080p.json")
+
+# FP32 VALU operations per event, counted on the reference lines the event
+# executes: ray generation (RaycastingKernels.cu:11-27, Ray.cuh:12-18), box test
+# (:718-757), sphere (:135-181), cylinder (:202-336), cube (:375-434), a
+# Compute decision (:620-653), and per hit the details (:183-200 etc.) plus
+# Phong (:71-105).
+FLOPS_RAYGEN, FLOPS_BOX, FLOPS_SPHERE, FLOPS_CYL, FLOPS_CUBE, FLOPS_COMPUTE = 50, 26, 37, 90, 59, 2
+FLOPS_HIT = 25 + 87
+PEAK_FP32_TFLOPS = 157.3
+
+
+def flops_per_ray(ev):
+    return (FLOPS_RAYGEN + FLOPS_BOX * ev["box"] + FLOPS_SPHERE * ev["sphere"] + FLOPS_CYL * ev["cylinder"]
+            + FLOPS_CUBE * ev["cube"] + FLOPS_COMPUTE * ev["compute"] + FLOPS_HIT * ev["hit"])
+
+
+def cpu_baseline(text, cam6, light3, phong8, w, h, row_step):
+    """The oracle's FP32 restatement (the reference's arithmetic) on all host
+    cores over every `row_step`-th image row."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    o = O.OracleScene(text)
+    threads = min(os.cpu_count() or 1, 16)
+    rows = len(range(0, h, row_step))
+    t0 = time.perf_counter()
+    o.render(cam6, light3, phong8, w, h, bits=32, threads=threads, rows=(0, h, row_step))
+    dt = time.perf_counter() - t0
+    return {"value": round(rows * w / dt / 1e6, 4), "unit": "Mrays/s", "cores": threads, "kind": "port",
+            "sample": f"every {row_step}th row of the {w}x{h} frame ({rows * w} rays), FP32 oracle, "
+                      f"{threads} threads, {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--mode", choices=("fast", "exact"), default="fast")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-row-step", type=int, default=16)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
+        sys.exit(2)
+
+    import torch
+    import csgray as C
+    from csgray import scenes as S
+
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    w, h = 1920, 1080
+    text = S.cheese(512, seed=512)
+    cam = C.camera_look_at(S.VIEW_EYE, S.VIEW_TARGET, S.VIEW_VFOV_DEG)
+    light = C.light(S.LIGHT_TOWARD)
+    ph = C.phong(*S.PHONG)
+    mode = C.MODE_FAST if args.mode == "fast" else C.MODE_EXACT
+    scene = C.Scene(text)
+    outs = {
+        "rgba": torch.empty((h, w, 4), dtype=torch.float32, device=dev),
+        "depth": torch.empty((h, w), dtype=torch.float32, device=dev),
+        "prim_id": torch.empty((h, w), dtype=torch.int32, device=dev),
+        "flags": torch.empty((h, w), dtype=torch.uint8, device=dev),
+    }
+    stream = torch.cuda.current_stream(dev)
+    for _ in range(args.warmup):
+        scene.render(cam, light, ph, w, h, mode=mode, outputs=outs)
+    torch.cuda.synchronize()
+
+    starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        starts[i].record(stream)
+        scene.render(cam, light, ph, w, h, mode=mode, outputs=outs)
+        ends[i].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    per = sorted(s.elapsed_time(e) for s, e in zip(starts, ends))  # ms, kernel-only (one launch per step)
+    total_ms = starts[0].elapsed_time(ends[-1])
+    rays = w * h
+    value = rays * args.steps / (total_ms / 1e3) / 1e6
+    p50 = per[len(per) // 2]
+    p99 = per[min(len(per) - 1, int(round(0.99 * (len(per) - 1))))]
+    avg_kernel_ms = sum(per) / len(per)
+
+    roofline = None
+    if os.path.exists(FLOP_FIXTURE):
+        with open(FLOP_FIXTURE) as f:
+            ev = json.load(f)["mean_events_per_ray"]
+        fpr = flops_per_ray(ev)
+        achieved = fpr * rays / (avg_kernel_ms / 1e3) / 1e12
+        traffic = None
+        if os.path.exists(TRAFFIC_FIXTURE):
+            with open(TRAFFIC_FIXTURE) as f:
+                traffic = json.load(f).get("hbm_bytes_per_launch")
+        roofline = {"bound": "valu_fp32", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
+                    "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": traffic,
+                    "flops_per_ray": round(fpr, 1), "kernel": "RenderKernel<fast>",
+                    "avg_kernel_ms": round(avg_kernel_ms, 4)}
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(text, C.cam_tuple(cam), S.LIGHT_TOWARD, S.PHONG + S.BACKGROUND, w, h,
+                           args.cpu_row_step)
+
+    line = {
+        "metric": "Mrays/s shaded primary rays (plain vs batched) + p50/p99 frame latency",
+        "value": round(value, 2), "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(total_ms / args.steps, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded scene)",
+        "config": {"workload": "plain: cheese 512 spheres, 1920x1080, 1 ray/pixel", "mode": args.mode,
+                   "rays_per_step": rays, "p50_ms": round(p50, 4), "p99_ms": round(p99, 4),
+                   "wall_s": round(wall, 3)},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,253 @@
+"""ctypes binding of libcsgray.so (the C ABI in include/csg.h).
+
+This is plumbing for tests, the benchmark and scripted renders: every call
+goes straight to the native library.  Device buffers come from torch (its
+allocator and its current stream); nothing here computes pixels.  If the
+library has not been built, `lib()` raises — there is no fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+
+_PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(_PKG_DIR, "lib", "libcsgray.so")
+
+c_float3 = ctypes.c_float * 3
+
+# csg_status
+OK, E_PARSE, E_DEPTH, E_OOM, E_HIP, E_ARG, E_BUDGET = range(7)
+STATUS_NAMES = {0: "CSG_OK", 1: "CSG_E_PARSE", 2: "CSG_E_DEPTH", 3: "CSG_E_OOM", 4: "CSG_E_HIP",
+                5: "CSG_E_ARG", 6: "CSG_E_BUDGET"}
+MODE_EXACT, MODE_FAST = 0, 1
+
+# hit flag bits (final record)
+ENTER, EXIT, MISS, FLIP, CAP_BOTTOM, CAP_TOP = 2, 4, 8, 16, 32, 64
+
+
+class Camera(ctypes.Structure):
+    _fields_ = [("pos", ctypes.c_float * 3), ("pitch", ctypes.c_float), ("yaw", ctypes.c_float),
+                ("vfov", ctypes.c_float)]
+
+
+class Light(ctypes.Structure):
+    _fields_ = [("dir", ctypes.c_float * 3)]
+
+
+class Phong(ctypes.Structure):
+    _fields_ = [("ka", ctypes.c_float), ("kd", ctypes.c_float), ("ks", ctypes.c_float),
+                ("shininess", ctypes.c_float), ("bg", ctypes.c_float * 4)]
+
+
+class Outputs(ctypes.Structure):
+    _fields_ = [("rgba", ctypes.c_void_p), ("depth", ctypes.c_void_p), ("prim_id", ctypes.c_void_p),
+                ("flags", ctypes.c_void_p)]
+
+
+class SceneInfo(ctypes.Structure):
+    _fields_ = [("nodes", ctypes.c_int32), ("primitives", ctypes.c_int32), ("op_depth", ctypes.c_int32),
+                ("spheres", ctypes.c_int32), ("cylinders", ctypes.c_int32), ("cubes", ctypes.c_int32),
+                ("device_bytes", ctypes.c_int64)]
+
+
+class RenderOpts(ctypes.Structure):
+    _fields_ = [("mode", ctypes.c_int32), ("block", ctypes.c_int32), ("rays_per_lane", ctypes.c_int32),
+                ("counters", ctypes.c_int32), ("counter_out", ctypes.c_void_p)]
+
+
+class CsgError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"{STATUS_NAMES.get(status, status)}: {msg}")
+        self.status = status
+
+
+_LIB = None
+
+# (name, restype, argtypes) for every function declared in include/csg.h
+_P = ctypes.c_void_p
+_SIGNATURES = [
+    ("csg_scene_parse", ctypes.c_int, [ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(_P), ctypes.c_char_p,
+                                       ctypes.c_size_t]),
+    ("csg_scene_parse_host", ctypes.c_int, [ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(_P),
+                                            ctypes.c_char_p, ctypes.c_size_t]),
+    ("csg_scene_upload", ctypes.c_int, [_P]),
+    ("csg_scene_free", ctypes.c_int, [_P]),
+    ("csg_scene_info_get", ctypes.c_int, [_P, ctypes.POINTER(SceneInfo)]),
+    ("csg_scene_export", ctypes.c_int, [_P, _P, _P, _P]),
+    ("csg_camera_default", None, [ctypes.POINTER(Camera)]),
+    ("csg_camera_look_at", ctypes.c_int, [c_float3, c_float3, ctypes.c_float, ctypes.POINTER(Camera)]),
+    ("csg_camera_basis", ctypes.c_int, [ctypes.POINTER(Camera), ctypes.c_float * 10]),
+    ("csg_light_default", None, [ctypes.POINTER(Light)]),
+    ("csg_light_from_angles", None, [ctypes.c_float, ctypes.c_float, ctypes.POINTER(Light)]),
+    ("csg_phong_default", None, [ctypes.POINTER(Phong)]),
+    ("csg_render", ctypes.c_int, [_P, ctypes.POINTER(Camera), ctypes.POINTER(Light), ctypes.POINTER(Phong),
+                                  ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(Outputs), _P]),
+    ("csg_render_ex", ctypes.c_int, [_P, ctypes.POINTER(Camera), ctypes.POINTER(Light), ctypes.POINTER(Phong),
+                                     ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(Outputs),
+                                     ctypes.POINTER(RenderOpts), _P]),
+    ("csg_sync", ctypes.c_int, [_P]),
+    ("csg_last_error", ctypes.c_char_p, []),
+    ("csg_version", ctypes.c_char_p, []),
+]
+
+
+def lib():
+    """Load libcsgray.so (raises if it has not been built)."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it first (python -c 'import __graft_entry__ as g; "
+                               "g.build()' or make -C cuda-csg-tree-raycasting_amd)")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, res, args in _SIGNATURES:
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def symbols():
+    return [s[0] for s in _SIGNATURES]
+
+
+def _check(status):
+    if status != OK:
+        raise CsgError(status, lib().csg_last_error().decode(errors="replace"))
+
+
+def camera_default():
+    c = Camera()
+    lib().csg_camera_default(ctypes.byref(c))
+    return c
+
+
+def camera_look_at(eye, target, vfov_deg):
+    c = Camera()
+    _check(lib().csg_camera_look_at(c_float3(*eye), c_float3(*target), float(vfov_deg), ctypes.byref(c)))
+    return c
+
+
+def camera(pos, pitch, yaw, vfov):
+    return Camera(c_float3(*pos), pitch, yaw, vfov)
+
+
+def camera_basis(cam):
+    out = (ctypes.c_float * 10)()
+    _check(lib().csg_camera_basis(ctypes.byref(cam), out))
+    return list(out)
+
+
+def light_default():
+    l = Light()
+    lib().csg_light_default(ctypes.byref(l))
+    return l
+
+
+def light_from_angles(polar, azimuth):
+    l = Light()
+    lib().csg_light_from_angles(float(polar), float(azimuth), ctypes.byref(l))
+    return l
+
+
+def light(direction):
+    return Light(c_float3(*direction))
+
+
+def phong_default():
+    p = Phong()
+    lib().csg_phong_default(ctypes.byref(p))
+    return p
+
+
+def phong(ka, kd, ks, shininess, bg=(0.08, 0.08, 0.11, 1.0)):
+    return Phong(ka, kd, ks, shininess, (ctypes.c_float * 4)(*bg))
+
+
+def cam_tuple(cam):
+    """(x, y, z, pitch, yaw, vfov) — the oracle's camera argument."""
+    return (cam.pos[0], cam.pos[1], cam.pos[2], cam.pitch, cam.yaw, cam.vfov)
+
+
+class Scene:
+    """A parsed scene (csg_scene*).  upload=False keeps it host-only."""
+
+    def __init__(self, text: str, upload: bool = True):
+        data = text.encode() if isinstance(text, str) else bytes(text)
+        h = _P()
+        err = ctypes.create_string_buffer(512)
+        fn = lib().csg_scene_parse if upload else lib().csg_scene_parse_host
+        st = fn(data, len(data), ctypes.byref(h), err, len(err))
+        if st != OK:
+            raise CsgError(st, err.value.decode(errors="replace"))
+        self.handle = h
+
+    def upload(self):
+        _check(lib().csg_scene_upload(self.handle))
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().csg_scene_free(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self) -> SceneInfo:
+        i = SceneInfo()
+        _check(lib().csg_scene_info_get(self.handle, ctypes.byref(i)))
+        return i
+
+    def export(self):
+        """(links int32[n,5], boxes float32[n,6], prims float32[p,14]) as numpy arrays."""
+        import numpy as np
+        i = self.info()
+        links = np.zeros((i.nodes, 5), np.int32)
+        boxes = np.zeros((i.nodes, 6), np.float32)
+        prims = np.zeros((i.primitives, 14), np.float32)
+        _check(lib().csg_scene_export(self.handle, links.ctypes.data, boxes.ctypes.data, prims.ctypes.data))
+        return links, boxes, prims
+
+    def render(self, cam, light_, phong_, width, height, mode=MODE_FAST, outputs=None, counters=False,
+               stream=None, device=None):
+        """Render into torch CUDA tensors; returns dict rgba/depth/prim_id/flags(/counters).
+
+        `outputs` may pass preallocated tensors (same keys) to avoid allocation."""
+        import torch
+        dev = device or torch.device("cuda", torch.cuda.current_device())
+        n = width * height
+        if outputs is None:
+            outputs = {
+                "rgba": torch.empty((height, width, 4), dtype=torch.float32, device=dev),
+                "depth": torch.empty((height, width), dtype=torch.float32, device=dev),
+                "prim_id": torch.empty((height, width), dtype=torch.int32, device=dev),
+                "flags": torch.empty((height, width), dtype=torch.uint8, device=dev),
+            }
+            if counters:
+                outputs["counters"] = torch.zeros((height, width, 6), dtype=torch.int32, device=dev)
+        for k, t in outputs.items():
+            assert t.is_cuda and t.is_contiguous(), k
+        assert outputs["rgba"].numel() >= 4 * n if "rgba" in outputs else True
+        o = Outputs(*(outputs[k].data_ptr() if outputs.get(k) is not None else None
+                      for k in ("rgba", "depth", "prim_id", "flags")))
+        opts = RenderOpts(mode, 0, 0, 1 if counters else 0,
+                          outputs["counters"].data_ptr() if counters else None)
+        s = stream if stream is not None else torch.cuda.current_stream(dev).cuda_stream
+        _check(lib().csg_render_ex(self.handle, ctypes.byref(cam), ctypes.byref(light_), ctypes.byref(phong_),
+                                   int(width), int(height), ctypes.byref(o), ctypes.byref(opts), s))
+        return outputs
+
+
+@dataclass
+class RenderSpec:
+    """One render job: scene text plus camera/light/shading and size."""
+    text: str
+    camera: Camera
+    light: Light
+    phong: Phong
+    width: int
+    height: int

@@ -1,0 +1,320 @@
+// C ABI (include/csg.h) over the host scene model and the HIP render kernel.
+#include "csg.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+
+#include "camera.hpp"
+#include "device.hpp"
+#include "scene.hpp"
+
+struct csg_scene {
+  csgray::CSGTree tree;
+  csgray::FlatScene flat;
+  int device = 0;
+  void* dmem = nullptr;
+  size_t bytes = 0;
+  csgray::DevScene dev{};
+};
+
+namespace {
+
+thread_local std::string g_lastError;
+
+csg_status Fail(csg_status s, const std::string& msg) {
+  g_lastError = msg;
+  return s;
+}
+
+csg_status FailHip(hipError_t e, const char* what) {
+  return Fail(CSG_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+void CopyErr(char* err, size_t errlen, const std::string& msg) {
+  if (err && errlen) std::snprintf(err, errlen, "%s", msg.c_str());
+}
+
+size_t Align256(size_t n) { return (n + 255) & ~static_cast<size_t>(255); }
+
+// One device allocation holding the five scene arrays back to back.
+csg_status Upload(csg_scene* s) {
+  const csgray::FlatScene& f = s->flat;
+  const size_t nb = Align256(sizeof(csgray::F4) * f.nNodes);
+  const size_t pb = Align256(sizeof(csgray::F4) * f.nPrims);
+  s->bytes = 2 * nb + 3 * pb;
+  hipError_t e = hipGetDevice(&s->device);
+  if (e != hipSuccess) return FailHip(e, "hipGetDevice");
+  e = hipMalloc(&s->dmem, s->bytes);
+  if (e != hipSuccess) {
+    s->dmem = nullptr;
+    return e == hipErrorOutOfMemory ? Fail(CSG_E_OOM, "hipMalloc of the scene failed") : FailHip(e, "hipMalloc");
+  }
+  char* base = static_cast<char*>(s->dmem);
+  char* parts[5] = {base, base + nb, base + 2 * nb, base + 2 * nb + pb, base + 2 * nb + 2 * pb};
+  const std::vector<csgray::F4>* src[5] = {&f.nodeLo, &f.nodeHi, &f.primA, &f.primB, &f.primC};
+  for (int k = 0; k < 5; k++) {
+    if (src[k]->empty()) continue;
+    e = hipMemcpy(parts[k], src[k]->data(), sizeof(csgray::F4) * src[k]->size(), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return FailHip(e, "hipMemcpy of the scene");
+  }
+  s->dev.nodeLo = reinterpret_cast<const float4*>(parts[0]);
+  s->dev.nodeHi = reinterpret_cast<const float4*>(parts[1]);
+  s->dev.primA = reinterpret_cast<const float4*>(parts[2]);
+  s->dev.primB = reinterpret_cast<const float4*>(parts[3]);
+  s->dev.primC = reinterpret_cast<const float4*>(parts[4]);
+  s->dev.nNodes = f.nNodes;
+  s->dev.nPrims = f.nPrims;
+  s->dev.opDepth = f.opDepth;
+  return CSG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* csg_version(void) { return "csgray 0.1 (gfx950)"; }
+
+const char* csg_last_error(void) { return g_lastError.c_str(); }
+
+csg_status csg_scene_parse_host(const char* text, size_t len, csg_scene** out, char* err, size_t errlen) {
+  if (!out) return Fail(CSG_E_ARG, "csg_scene_parse: out is NULL");
+  *out = nullptr;
+  if (!text && len) return Fail(CSG_E_ARG, "csg_scene_parse: text is NULL");
+  csg_scene* s = new (std::nothrow) csg_scene();
+  if (!s) return Fail(CSG_E_OOM, "csg_scene_parse: host allocation failed");
+  try {
+    s->tree = csgray::CSGTree::Parse(std::string(text ? text : "", len));
+  } catch (const csgray::ParseError& e) {
+    delete s;
+    CopyErr(err, errlen, e.what());
+    return Fail(CSG_E_PARSE, e.what());
+  } catch (const std::bad_alloc&) {
+    delete s;
+    CopyErr(err, errlen, "out of host memory");
+    return Fail(CSG_E_OOM, "csg_scene_parse: host allocation failed");
+  }
+  const int32_t nn = static_cast<int32_t>(s->tree.nodes.size());
+  const int32_t np = static_cast<int32_t>(s->tree.primitives.size());
+  std::string limit;
+  csg_status code = CSG_OK;
+  if (nn > csgray::kMaxNodes) {
+    limit = "scene has " + std::to_string(nn) + " nodes; the limit is " + std::to_string(csgray::kMaxNodes);
+    code = CSG_E_ARG;
+  } else if (np > csgray::kMaxPrimitives) {
+    limit = "scene has " + std::to_string(np) + " primitives; hit ids are int16, the limit is " +
+            std::to_string(csgray::kMaxPrimitives);
+    code = CSG_E_ARG;
+  } else if (s->tree.opDepth > csgray::MaxSupportedDepth()) {
+    limit = "operator depth " + std::to_string(s->tree.opDepth) + " exceeds the supported " +
+            std::to_string(csgray::MaxSupportedDepth());
+    code = CSG_E_DEPTH;
+  }
+  if (code != CSG_OK) {
+    delete s;
+    CopyErr(err, errlen, limit);
+    return Fail(code, limit);
+  }
+  s->flat = csgray::Flatten(s->tree);
+  *out = s;
+  return CSG_OK;
+}
+
+csg_status csg_scene_upload(csg_scene* s) {
+  if (!s) return Fail(CSG_E_ARG, "csg_scene_upload: NULL scene");
+  if (s->dmem) return CSG_OK;
+  const csg_status st = Upload(s);
+  if (st != CSG_OK && s->dmem) {
+    (void)hipFree(s->dmem);
+    s->dmem = nullptr;
+  }
+  return st;
+}
+
+csg_status csg_scene_parse(const char* text, size_t len, csg_scene** out, char* err, size_t errlen) {
+  csg_status st = csg_scene_parse_host(text, len, out, err, errlen);
+  if (st != CSG_OK) return st;
+  st = csg_scene_upload(*out);
+  if (st != CSG_OK) {
+    CopyErr(err, errlen, g_lastError);
+    const std::string keep = g_lastError;
+    delete *out;
+    *out = nullptr;
+    g_lastError = keep;
+  }
+  return st;
+}
+
+csg_status csg_scene_free(csg_scene* scene) {
+  if (!scene) return CSG_OK;
+  csg_status st = CSG_OK;
+  if (scene->dmem) {
+    const hipError_t e = hipFree(scene->dmem);
+    if (e != hipSuccess) st = FailHip(e, "hipFree");
+  }
+  delete scene;
+  return st;
+}
+
+csg_status csg_scene_info_get(const csg_scene* s, csg_scene_info* info) {
+  if (!s || !info) return Fail(CSG_E_ARG, "csg_scene_info_get: NULL argument");
+  std::memset(info, 0, sizeof(*info));
+  info->nodes = s->flat.nNodes;
+  info->primitives = s->flat.nPrims;
+  info->op_depth = s->flat.opDepth;
+  for (const auto& p : s->tree.primitives) {
+    if (p.type == csgray::kSphere) info->spheres++;
+    else if (p.type == csgray::kCylinder) info->cylinders++;
+    else info->cubes++;
+  }
+  info->device_bytes = static_cast<int64_t>(s->bytes);
+  return CSG_OK;
+}
+
+csg_status csg_scene_export(const csg_scene* s, int32_t* links, float* boxes, float* prims) {
+  if (!s) return Fail(CSG_E_ARG, "csg_scene_export: NULL scene");
+  const auto& nodes = s->tree.nodes;
+  for (size_t i = 0; i < nodes.size(); i++) {
+    if (links) {
+      int32_t* l = links + 5 * i;
+      l[0] = nodes[i].type; l[1] = nodes[i].primitiveIdx; l[2] = nodes[i].left;
+      l[3] = nodes[i].right; l[4] = nodes[i].parent;
+    }
+    if (boxes) {
+      float* b = boxes + 6 * i;
+      for (int k = 0; k < 3; k++) {
+        b[k] = nodes[i].box.lo[k];
+        b[3 + k] = nodes[i].box.hi[k];
+      }
+    }
+  }
+  if (prims) {
+    const auto& ps = s->tree.primitives;
+    for (size_t i = 0; i < ps.size(); i++) {
+      float* o = prims + 14 * i;
+      const auto& p = ps[i];
+      o[0] = static_cast<float>(p.id); o[1] = static_cast<float>(p.type);
+      o[2] = p.x; o[3] = p.y; o[4] = p.z; o[5] = p.r; o[6] = p.g; o[7] = p.b;
+      o[8] = p.radius; o[9] = p.height; o[10] = p.size;
+      o[11] = p.axis[0]; o[12] = p.axis[1]; o[13] = p.axis[2];
+    }
+  }
+  return CSG_OK;
+}
+
+void csg_camera_default(csg_camera* cam) {  // Camera() : Camera.h:18-20
+  if (!cam) return;
+  cam->pos[0] = 0; cam->pos[1] = 0; cam->pos[2] = 5;
+  cam->pitch = 0; cam->yaw = 0;
+  cam->vfov = 90.0f * 3.14159f / 180.0f;
+}
+
+csg_status csg_camera_look_at(const float eye[3], const float target[3], float vfov_deg, csg_camera* cam) {
+  if (!eye || !target || !cam) return Fail(CSG_E_ARG, "csg_camera_look_at: NULL argument");
+  // forward = (-sin yaw cos pitch, sin pitch, -cos yaw cos pitch) (Camera.cpp:10-12)
+  const double fx = static_cast<double>(target[0]) - eye[0];
+  const double fy = static_cast<double>(target[1]) - eye[1];
+  const double fz = static_cast<double>(target[2]) - eye[2];
+  const double len = std::sqrt(fx * fx + fy * fy + fz * fz);
+  if (!(len > 0)) return Fail(CSG_E_ARG, "csg_camera_look_at: eye equals target");
+  cam->pos[0] = eye[0]; cam->pos[1] = eye[1]; cam->pos[2] = eye[2];
+  const float lim = csgray::PitchLimit();
+  const float pitch = static_cast<float>(std::asin(fy / len));
+  cam->pitch = std::fmax(-lim, std::fmin(lim, pitch));
+  cam->yaw = static_cast<float>(std::atan2(-fx, -fz));
+  cam->vfov = vfov_deg * 3.14159f / 180.0f;  // Camera::setFOV, Camera.h:49-52
+  return CSG_OK;
+}
+
+csg_status csg_camera_basis(const csg_camera* cam, float out10[10]) {
+  if (!cam || !out10) return Fail(CSG_E_ARG, "csg_camera_basis: NULL argument");
+  const csgray::CameraBasis b = csgray::MakeBasis(cam->pos, cam->pitch, cam->yaw, cam->vfov);
+  for (int k = 0; k < 3; k++) {
+    out10[k] = b.forward[k];
+    out10[3 + k] = b.right[k];
+    out10[6 + k] = b.up[k];
+  }
+  out10[9] = b.tanHalf;
+  return CSG_OK;
+}
+
+void csg_light_default(csg_light* light) {  // DirectionalLight.h:10-11
+  if (!light) return;
+  csgray::LightFromAngles(-60.f * 3.14159f / 180.f, -45.f * 3.14159f / 180.f, light->dir);
+}
+
+void csg_light_from_angles(float polar, float azimuth, csg_light* light) {
+  if (!light) return;
+  csgray::LightFromAngles(polar, azimuth, light->dir);
+}
+
+void csg_phong_default(csg_phong* ph) {  // RaycastingKernels.cu:71-74, 109
+  if (!ph) return;
+  ph->ka = 0.2f; ph->kd = 0.8f; ph->ks = 0.7f; ph->shininess = 30.0f;
+  ph->bg[0] = 0.08f; ph->bg[1] = 0.08f; ph->bg[2] = 0.11f; ph->bg[3] = 1.0f;
+}
+
+csg_status csg_render_ex(const csg_scene* s, const csg_camera* cam, const csg_light* light, const csg_phong* phong,
+                         int32_t width, int32_t height, const csg_outputs* out, const csg_render_opts* opts,
+                         void* stream) {
+  if (!s || !cam || !light || !phong || !out) return Fail(CSG_E_ARG, "csg_render: NULL argument");
+  // The reference's ray generation divides by width-1 and height-1
+  // (RaycastingKernels.cu:11-12), so both must be at least 2.
+  if (width < 2 || height < 2) return Fail(CSG_E_ARG, "csg_render: width and height must be >= 2");
+  if (static_cast<int64_t>(width) * height > (int64_t{1} << 31))
+    return Fail(CSG_E_ARG, "csg_render: image larger than 2^31 pixels");
+  csgray::LaunchConfig cfg;
+  if (opts) {
+    if (opts->mode != CSG_MODE_EXACT && opts->mode != CSG_MODE_FAST)
+      return Fail(CSG_E_ARG, "csg_render: unknown mode");
+    cfg.mode = opts->mode == CSG_MODE_EXACT ? csgray::Mode::kExact : csgray::Mode::kFast;
+    cfg.count = opts->counters != 0;
+    if (cfg.count && !opts->counter_out) return Fail(CSG_E_ARG, "csg_render: counters requested without a buffer");
+  }
+  if (!s->dmem) return Fail(CSG_E_ARG, "csg_render: scene is not on a device (call csg_scene_upload)");
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return FailHip(e, "hipGetDevice");
+  if (dev != s->device) return Fail(CSG_E_ARG, "csg_render: scene was uploaded to another device");
+
+  csgray::RenderParams p{};
+  p.sc = s->dev;
+  const csgray::CameraBasis b = csgray::MakeBasis(cam->pos, cam->pitch, cam->yaw, cam->vfov);
+  for (int k = 0; k < 3; k++) {
+    p.pos[k] = b.pos[k];
+    p.forward[k] = b.forward[k];
+    p.right[k] = b.right[k];
+    p.up[k] = b.up[k];
+  }
+  p.tanHalf = b.tanHalf;
+  csgray::NormalizeLight(light->dir, p.light);
+  p.ka = phong->ka; p.kd = phong->kd; p.ks = phong->ks; p.shininess = phong->shininess;
+  for (int k = 0; k < 4; k++) p.bg[k] = phong->bg[k];
+  p.width = width;
+  p.height = height;
+  p.rgba = reinterpret_cast<float4*>(out->rgba);
+  p.depth = out->depth;
+  p.primId = out->prim_id;
+  p.flags = out->flags;
+  p.counters = cfg.count ? opts->counter_out : nullptr;
+  e = csgray::LaunchRender(p, cfg, static_cast<hipStream_t>(stream));
+  if (e != hipSuccess) return FailHip(e, "render launch");
+  return CSG_OK;
+}
+
+csg_status csg_render(const csg_scene* s, const csg_camera* cam, const csg_light* light, const csg_phong* phong,
+                      int32_t width, int32_t height, const csg_outputs* out, void* stream) {
+  return csg_render_ex(s, cam, light, phong, width, height, out, nullptr, stream);
+}
+
+csg_status csg_sync(void* stream) {
+  const hipError_t e = hipStreamSynchronize(static_cast<hipStream_t>(stream));
+  if (e != hipSuccess) return FailHip(e, "hipStreamSynchronize");
+  return CSG_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,264 @@
+// Preorder-text scene loader (see scene.hpp).
+#include "scene.hpp"
+
+#include <algorithm>
+#include <cctype>
+#include <cmath>
+#include <cstring>
+#include <string_view>
+
+namespace csgray {
+
+namespace {
+
+// Whitespace tokeniser with the semantics of `stringstream >> string`
+// (CSGTree.cu:181-194): any run of isspace() characters separates tokens.
+std::vector<std::string> Tokens(const std::string& text) {
+  std::vector<std::string> out;
+  size_t i = 0, n = text.size();
+  while (i < n) {
+    while (i < n && std::isspace(static_cast<unsigned char>(text[i]))) i++;
+    size_t j = i;
+    while (j < n && !std::isspace(static_cast<unsigned char>(text[j]))) j++;
+    if (j > i) out.emplace_back(text, i, j - i);
+    i = j;
+  }
+  return out;
+}
+
+// Number conversions keep std::stof / std::stod / std::stoi semantics
+// (leading-prefix parse; "no digits" and "out of range" are errors), which is
+// what the reference calls (CSGTree.cu:62-132, 196-200).
+float ToFloat(const std::string& s) {
+  try {
+    return std::stof(s);
+  } catch (const std::exception& e) {
+    throw ParseError(std::string("Cannot parse number '") + s + "': " + e.what());
+  }
+}
+
+double ToDouble(const std::string& s) {
+  try {
+    return std::stod(s);
+  } catch (const std::exception& e) {
+    throw ParseError(std::string("Cannot parse number '") + s + "': " + e.what());
+  }
+}
+
+// One colour channel: two hex digits scaled to [0, 1] (CSGTree.cu:196-200).
+float Channel(const std::string& two) {
+  int v;
+  try {
+    v = std::stoi(two, nullptr, 16);
+  } catch (const std::exception& e) {
+    throw ParseError(std::string("Cannot parse color channel '") + two + "': " + e.what());
+  }
+  return static_cast<float>(v) / 255;
+}
+
+void Colour(const std::string& hex, Primitive& p) {
+  if (hex.size() != 6) throw ParseError("Cannot parse color " + hex);
+  p.r = Channel(hex.substr(0, 2));
+  p.g = Channel(hex.substr(2, 2));
+  p.b = Channel(hex.substr(4, 2));
+}
+
+// Cylinder axis = R * (0, 1, 0) for Euler angles in degrees, evaluated in
+// double and divided by its squared length before rounding to float
+// (Primitives.h:66-87).
+void CylinderAxis(Primitive& p, double degX, double degY, double degZ) {
+  const double k = 0.017453292519943295769236907684886;
+  const double rx = degX * k, ry = degY * k, rz = degZ * k;
+  const double sx = std::sin(rx), cx = std::cos(rx);
+  const double sy = std::sin(ry), cy = std::cos(ry);
+  const double sz = std::sin(rz), cz = std::cos(rz);
+  double v0 = -sz * cy + sy * sx * cz;
+  double v1 = cx * cz;
+  double v2 = sy * sz + sx * cy * cz;
+  const double sq = v0 * v0 + v1 * v1 + v2 * v2;
+  v0 /= sq;
+  v1 /= sq;
+  v2 /= sq;
+  p.axis[0] = static_cast<float>(v0);
+  p.axis[1] = static_cast<float>(v1);
+  p.axis[2] = static_cast<float>(v2);
+}
+
+void CheckDegrees(double v, const char* name) {
+  if (v > 360 || v < 0)
+    throw ParseError(std::string("Invalid roation ") + name + " should be in range [0, 360] deg");
+}
+
+Box LeafBox(const Primitive& p) {  // BVHNode.cuh:17-65
+  Box b;
+  const float c[3] = {p.x, p.y, p.z};
+  if (p.type == kSphere) {
+    // 2r half-extent, two separate float subtractions as in the reference
+    for (int k = 0; k < 3; k++) {
+      b.lo[k] = c[k] - p.radius - p.radius;
+      b.hi[k] = c[k] + p.radius + p.radius;
+    }
+  } else if (p.type == kCylinder) {
+    // max(h/2, r) about the centre whatever the axis (the reference's box;
+    // not conservative for tilted axes, reproduced as is)
+    const float e = std::max(p.height / 2, p.radius);
+    for (int k = 0; k < 3; k++) {
+      b.lo[k] = c[k] - e;
+      b.hi[k] = c[k] + e;
+    }
+  } else {
+    const float e = p.size / 2;
+    for (int k = 0; k < 3; k++) {
+      b.lo[k] = c[k] - e;
+      b.hi[k] = c[k] + e;
+    }
+  }
+  return b;
+}
+
+}  // namespace
+
+CSGTree CSGTree::Parse(const std::string& text) {
+  CSGTree tree;
+  const std::vector<std::string> tok = Tokens(text);
+
+  // Open operators waiting for children: (node index, children attached).
+  std::vector<std::pair<int32_t, int32_t>> open;
+  int32_t nPrims = 0;
+
+  size_t i = 0;
+  while (i < tok.size()) {
+    const int32_t self = static_cast<int32_t>(tree.nodes.size());
+    tree.nodes.emplace_back();
+    if (self != 0) {
+      if (open.empty()) throw ParseError("Cannot parse");  // tokens after a complete tree
+      auto& top = open.back();
+      tree.nodes[self].parent = top.first;
+      if (top.second == 0) {
+        tree.nodes[top.first].left = self;
+        top.second = 1;
+      } else {
+        tree.nodes[top.first].right = self;
+        open.pop_back();
+      }
+    }
+
+    const std::string& key = tok[i];
+    CSGNode& node = tree.nodes[self];
+    int32_t args = 0;
+    if (key == "Union") {
+      node.type = kUnion;
+    } else if (key == "Difference") {
+      node.type = kDifference;
+    } else if (key == "Intersection") {
+      node.type = kIntersection;
+    } else if (key == "Sphere" || key == "Cube") {
+      args = 5;
+    } else if (key == "Cylinder") {
+      args = 9;
+    } else {
+      throw ParseError("Cannot parse - Unrecognized keyword: " + key);
+    }
+
+    if (args == 0) {
+      open.push_back({self, 0});
+      i += 1;
+    } else {
+      if (i + args >= tok.size()) throw ParseError("Cannot parse - truncated primitive: " + key);
+      Primitive p;
+      p.id = nPrims;
+      p.type = key == "Sphere" ? kSphere : (key == "Cube" ? kCube : kCylinder);
+      p.x = ToFloat(tok[i + 1]);
+      p.y = ToFloat(tok[i + 2]);
+      p.z = ToFloat(tok[i + 3]);
+      Colour(tok[i + 4], p);
+      if (p.type == kSphere) {
+        p.radius = ToFloat(tok[i + 5]);
+      } else if (p.type == kCube) {
+        p.size = ToFloat(tok[i + 5]);
+      } else {
+        p.radius = ToFloat(tok[i + 5]);
+        p.height = ToFloat(tok[i + 6]);
+        const double rx = ToDouble(tok[i + 7]);
+        const double ry = ToDouble(tok[i + 8]);
+        const double rz = ToDouble(tok[i + 9]);
+        CheckDegrees(rx, "rotX");
+        CheckDegrees(ry, "rotY");
+        CheckDegrees(rz, "rotZ");
+        CylinderAxis(p, rx, ry, rz);
+      }
+      node.type = p.type;
+      node.primitiveIdx = nPrims;
+      tree.primitives.push_back(p);
+      nPrims++;
+      i += 1 + args;
+    }
+  }
+
+  const int32_t nNodes = static_cast<int32_t>(tree.nodes.size());
+  if (nNodes != 2 * nPrims - 1)
+    throw ParseError("Cannot parse - number of primitives do not match number of nodes");
+
+  tree.ConstructBVH();
+  return tree;
+}
+
+void CSGTree::ConstructBVH() {
+  // Children follow their parent in preorder, so one reverse sweep sees every
+  // child before its parent (CSGTree.cu:154-179 does the same post-order with
+  // an explicit stack).  Operator depth comes out of the same sweep.
+  std::vector<int32_t> depth(nodes.size(), 0);
+  for (size_t k = nodes.size(); k-- > 0;) {
+    CSGNode& n = nodes[k];
+    if (n.primitiveIdx != -1) {
+      n.box = LeafBox(primitives[n.primitiveIdx]);
+      depth[k] = 0;
+    } else {
+      const Box& L = nodes[n.left].box;
+      const Box& R = nodes[n.right].box;
+      for (int a = 0; a < 3; a++) {
+        n.box.lo[a] = std::min(L.lo[a], R.lo[a]);
+        n.box.hi[a] = std::max(L.hi[a], R.hi[a]);
+      }
+      depth[k] = 1 + std::max(depth[n.left], depth[n.right]);
+    }
+  }
+  opDepth = nodes.empty() ? 0 : depth[0];
+}
+
+FlatScene Flatten(const CSGTree& tree) {
+  FlatScene f;
+  f.nNodes = static_cast<int32_t>(tree.nodes.size());
+  f.nPrims = static_cast<int32_t>(tree.primitives.size());
+  f.opDepth = tree.opDepth;
+  auto bits = [](uint32_t u) {
+    float v;
+    std::memcpy(&v, &u, 4);
+    return v;
+  };
+  f.nodeLo.resize(f.nNodes);
+  f.nodeHi.resize(f.nNodes);
+  for (int32_t i = 0; i < f.nNodes; i++) {
+    const CSGNode& n = tree.nodes[i];
+    const uint32_t prim = n.primitiveIdx < 0 ? 0xFFFFFFu : static_cast<uint32_t>(n.primitiveIdx);
+    const uint32_t meta = static_cast<uint32_t>(n.type) | (prim << 8);
+    const uint32_t right = n.right < 0 ? 0xFFFFu : static_cast<uint32_t>(n.right);
+    const uint32_t parent = n.parent < 0 ? 0xFFFFu : static_cast<uint32_t>(n.parent);
+    const uint32_t link = right | (parent << 16);
+    f.nodeLo[i] = F4{n.box.lo[0], n.box.lo[1], n.box.lo[2], bits(meta)};
+    f.nodeHi[i] = F4{n.box.hi[0], n.box.hi[1], n.box.hi[2], bits(link)};
+  }
+  f.primA.resize(f.nPrims);
+  f.primB.resize(f.nPrims);
+  f.primC.resize(f.nPrims);
+  for (int32_t p = 0; p < f.nPrims; p++) {
+    const Primitive& q = tree.primitives[p];
+    const float s = q.type == kCube ? q.size : q.radius;
+    f.primA[p] = F4{q.x, q.y, q.z, s};
+    f.primB[p] = F4{q.axis[0], q.axis[1], q.axis[2], q.height};
+    f.primC[p] = F4{q.r, q.g, q.b, 0.0f};
+  }
+  return f;
+}
+
+}  // namespace csgray

@@ -1,0 +1,87 @@
+// Host scene model: the preorder-text loader and its flattening into the
+// device layout the traversal kernels read.
+//
+// Mirrors the reference's CSGTree (Graphics/RayCasting/CSGTree/CSGTree.cuh:15-65),
+// Primitive (CSGTree/Primitives/Primitives.h:29-102) and BVHNode
+// (CSGTree/BVH/BVHNode.cuh:7-87).
+#pragma once
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace csgray {
+
+// Node kinds in the reference's numbering (CSGTree.cuh:38-46); the kernels
+// test `type < kSphere` for "operator".
+enum NodeType : int32_t {
+  kUnion = 0,
+  kDifference = 1,
+  kIntersection = 2,
+  kSphere = 3,
+  kCylinder = 4,
+  kCube = 5,
+};
+
+// Raised for every input the reference's parser rejects.
+struct ParseError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+struct Primitive {
+  int32_t id = 0;
+  int32_t type = kSphere;
+  float x = 0, y = 0, z = 0;
+  float r = 0, g = 0, b = 0;
+  float radius = 0;  // sphere, cylinder
+  float height = 0;  // cylinder
+  float size = 0;    // cube: full edge length
+  float axis[3] = {0, 0, 0};  // cylinder: R * (0,1,0), computed in double
+};
+
+struct Box {
+  float lo[3] = {0, 0, 0};
+  float hi[3] = {0, 0, 0};
+};
+
+struct CSGNode {
+  int32_t type = -1;
+  int32_t primitiveIdx = -1;  // -1 for operators
+  int32_t left = -1;          // always index + 1 for operators (preorder)
+  int32_t right = -1;
+  int32_t parent = -1;        // -1 for the root
+  Box box;
+};
+
+struct CSGTree {
+  std::vector<CSGNode> nodes;  // node 0 is the root, preorder
+  std::vector<Primitive> primitives;
+  int32_t opDepth = 0;
+
+  // Throws ParseError (and nothing else) on input the reference rejects.
+  static CSGTree Parse(const std::string& text);
+  void ConstructBVH();
+};
+
+// Device layout, 16-byte records (see the kernels for the readers):
+//   nodeLo[i] = (box.lo.xyz, bits(meta))   meta = type | (primitive << 8)
+//   nodeHi[i] = (box.hi.xyz, bits(link))   link = right | (parent << 16), parent 0xFFFF at the root
+//   primA[p]  = (centre.xyz, radius or cube edge)
+//   primB[p]  = (axis.xyz, height)
+//   primC[p]  = (r, g, b, 0)
+struct alignas(16) F4 {
+  float x, y, z, w;
+};
+
+struct FlatScene {
+  std::vector<F4> nodeLo, nodeHi, primA, primB, primC;
+  int32_t nNodes = 0, nPrims = 0, opDepth = 0;
+};
+
+constexpr int32_t kMaxNodes = 65535;       // 16-bit node links
+constexpr int32_t kMaxPrimitives = 32767;  // the reference stores hit ids as int16 (Ray.cuh:41)
+
+FlatScene Flatten(const CSGTree& tree);
+
+}  // namespace csgray

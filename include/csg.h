@@ -1,0 +1,153 @@
+/* csg.h — C ABI of the MI355X CSG ray caster (libcsgray.so).
+ *
+ * The reference exposes its render path as two C++ entry points, and this
+ * header is the drop-in for them:
+ *   CSGTree::Parse(const std::string&)          Graphics/RayCasting/CSGTree/CSGTree.cuh:53,
+ *                                               CSGTree.cu:5-152 (parse + ConstructBVH)
+ *   Raycaster::ChangeSize(int, int, CSGTree)    Graphics/RayCasting/Raycaster.cuh:21, Raycaster.cu:3-21
+ *   Raycaster::Raycast(float4*, Camera, DirectionalLight)
+ *                                               Raycaster.cuh:22, Raycaster.cu:23-34
+ *   Raycaster::CleanUp()                        Raycaster.cuh:23, Raycaster.cu:36-45
+ * plus the value types it passes by value: Camera (RenderManager/Camera/Camera.h:3-54)
+ * and DirectionalLight (RenderManager/DirectionalLight.h:8-18).
+ *
+ * Conventions
+ *   - Every function returns a csg_status; none throws or exits.  The text of
+ *     the last error on the calling thread is available from csg_last_error().
+ *   - Scenes are immutable after csg_scene_parse and may be rendered
+ *     concurrently from several host threads and streams.
+ *   - Output pointers are caller-owned device pointers (any may be NULL).  The
+ *     image is row-major, pixel (x, y) at index y*width + x; row 0 is the
+ *     bottom scanline when displayed (its rays point down), as in the reference.
+ *   - `stream` is a hipStream_t passed as void* (NULL = the null stream).  All
+ *     renders are stream-ordered and asynchronous; csg_sync waits.
+ */
+#ifndef CSG_H_
+#define CSG_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum csg_status {
+  CSG_OK = 0,
+  CSG_E_PARSE = 1, /* scene text rejected (the reference throws std::invalid_argument) */
+  CSG_E_DEPTH = 2, /* operator depth above what the traversal stack holds */
+  CSG_E_OOM = 3,   /* device or host allocation failed */
+  CSG_E_HIP = 4,   /* a HIP runtime call failed */
+  CSG_E_ARG = 5,   /* invalid argument (null pointer, size, range) */
+  CSG_E_BUDGET = 6 /* serving: request larger than the admission budget */
+} csg_status;
+
+typedef struct csg_scene csg_scene;
+
+/* Camera: the reference's pitch/yaw model (Camera.h:9-16).  pitch = rotX,
+ * yaw = rotY, both in radians; vfov = vertical field of view in radians.
+ * The basis is forward = (-sin yaw cos pitch, sin pitch, -cos yaw cos pitch),
+ * right = (cos yaw, 0, -sin yaw), up = forward x right (Camera.cpp:4-27);
+ * pitch is clamped to +-89 degrees as Camera::setRotation does. */
+typedef struct csg_camera {
+  float pos[3];
+  float pitch;
+  float yaw;
+  float vfov;
+} csg_camera;
+
+/* Directional light: `dir` points TOWARD the light, as the reference's
+ * getLightDir() result is used (RaycastingKernels.cu:78-86).  It need not be
+ * normalised. */
+typedef struct csg_light {
+  float dir[3];
+} csg_light;
+
+/* Phong constants.  The reference hard-codes ka 0.2, kd 0.8, ks 0.7,
+ * shininess 30 and background (0.08, 0.08, 0.11, 1)
+ * (RaycastingKernels.cu:71-74,109); csg_phong_default returns those. */
+typedef struct csg_phong {
+  float ka, kd, ks, shininess;
+  float bg[4];
+} csg_phong;
+
+/* Per-pixel outputs (device pointers, each may be NULL):
+ *   rgba    float4 per pixel, the reference's RGBA32F framebuffer
+ *   depth   hit distance t along the unit ray (+inf on a miss)
+ *   prim_id index of the visible primitive in file order (-1 on a miss)
+ *   flags   the final hit bits: 2 enter, 4 exit, 8 miss, 16 flipped,
+ *           32 bottom cap, 64 top cap (Utils/CSGUtils.cuh:27-35) */
+typedef struct csg_outputs {
+  float* rgba;
+  float* depth;
+  int32_t* prim_id;
+  uint8_t* flags;
+} csg_outputs;
+
+typedef struct csg_scene_info {
+  int32_t nodes;
+  int32_t primitives;
+  int32_t op_depth;  /* operator levels above the deepest leaf */
+  int32_t spheres, cylinders, cubes;
+  int64_t device_bytes;
+} csg_scene_info;
+
+/* Arithmetic of the traversal kernel.
+ *   CSG_MODE_EXACT: the reference's FP32 expressions as written — IEEE
+ *     division and square root, no fused multiply-add — so hit id, depth and
+ *     flags are bit-identical to a host FP32 build of the reference logic.
+ *   CSG_MODE_FAST: box slabs from a per-ray reciprocal direction, and fused
+ *     multiply-add where the compiler finds it; decisions stay the reference's
+ *     and the result is checked against the FP64 host oracle. */
+typedef enum csg_mode { CSG_MODE_EXACT = 0, CSG_MODE_FAST = 1 } csg_mode;
+
+typedef struct csg_render_opts {
+  int32_t mode;         /* csg_mode */
+  int32_t block;        /* threads per workgroup: 0 = auto, else 64..1024, multiple of 64 */
+  int32_t rays_per_lane;/* persistent refill depth: 0 = auto, 1 = one ray per lane */
+  int32_t counters;     /* 1: also write per-ray event counters (testing) */
+  int32_t* counter_out; /* device, 6 int32 per pixel: GoTo, Compute, box, sphere, cylinder, cube */
+} csg_render_opts;
+
+/* --- scene (replaces CSGTree::Parse + the upload in Raycaster::ChangeSize) --- */
+/* Parse, build boxes, flatten and upload to the current device. */
+csg_status csg_scene_parse(const char* text, size_t len, csg_scene** out, char* err, size_t errlen);
+/* The host half alone (CSGTree::Parse + ConstructBVH): no device memory is
+ * touched; csg_scene_upload later copies the scene to the current device. */
+csg_status csg_scene_parse_host(const char* text, size_t len, csg_scene** out, char* err, size_t errlen);
+csg_status csg_scene_upload(csg_scene* scene);
+csg_status csg_scene_free(csg_scene* scene);
+csg_status csg_scene_info_get(const csg_scene* scene, csg_scene_info* info);
+/* Host copy of the flattened scene, for tests: links = 5 int32 per node
+ * (type, primitive, left, right, parent), boxes = 6 float per node
+ * (min xyz, max xyz), prims = 14 float per primitive (id, type, x, y, z,
+ * r, g, b, radius, height, size, axis xyz).  Any pointer may be NULL. */
+csg_status csg_scene_export(const csg_scene* scene, int32_t* links, float* boxes, float* prims);
+
+/* --- camera / light / shading helpers --- */
+void csg_camera_default(csg_camera* cam);           /* Camera() : (0,0,5), pitch=yaw=0, fov 90 deg */
+/* eye -> target, vfov in degrees converted with 3.14159 as Camera::setFOV does */
+csg_status csg_camera_look_at(const float eye[3], const float target[3], float vfov_deg, csg_camera* cam);
+/* forward[3], right[3], up[3], tan(vfov/2) as the render computes them */
+csg_status csg_camera_basis(const csg_camera* cam, float out10[10]);
+void csg_light_default(csg_light* light);           /* polar -60 deg, azimuth -45 deg */
+void csg_light_from_angles(float polar, float azimuth, csg_light* light);
+void csg_phong_default(csg_phong* phong);
+
+/* --- render (replaces Raycaster::Raycast: RaycastKernel + LightningKernel) --- */
+csg_status csg_render(const csg_scene* scene, const csg_camera* cam, const csg_light* light,
+                      const csg_phong* phong, int32_t width, int32_t height, const csg_outputs* out,
+                      void* stream);
+csg_status csg_render_ex(const csg_scene* scene, const csg_camera* cam, const csg_light* light,
+                         const csg_phong* phong, int32_t width, int32_t height, const csg_outputs* out,
+                         const csg_render_opts* opts, void* stream);
+csg_status csg_sync(void* stream);
+
+const char* csg_last_error(void);
+const char* csg_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CSG_H_ */

@@ -1,0 +1,127 @@
+"""GPU parity: the HIP render (through the C ABI) against the oracle.
+
+Exact mode must be bit-identical to the oracle's FP32 build (hit id, depth
+bits, flags, and the per-ray event counters of the state machine); RGBA may
+differ by the ULPs of the device powf/sqrtf vs host libm.  Fast mode is held
+to the FP64 oracle: hit id and enter/exit/miss agree on >= 99.99% of the rays
+the FP64 run does not flag as degenerate, and depth is within 1e-4 relative.
+"""
+import numpy as np
+import pytest
+import torch
+
+import csgray as C
+import oracle as O
+from common import setups
+from conftest import scene_names, scene_text
+
+pytestmark = pytest.mark.gpu
+
+RGBA_TOL = 2e-6        # device vs host powf in the specular term
+DEPTH_REL_TOL = 1e-4   # fast mode vs FP64
+AGREE_MIN = 0.9999     # hit-id agreement on unflagged rays
+
+
+def gpu_render(text, cam, light, phong, w, h, mode, counters=False):
+    s = C.Scene(text)
+    out = s.render(cam, light, phong, w, h, mode=mode, counters=counters)
+    torch.cuda.synchronize()
+    res = {k: v.cpu().numpy() for k, v in out.items()}
+    s.close()
+    return res
+
+
+def _label(name, setup):
+    print(f"\n[case] {name} setup={setup}", flush=True)
+
+
+@pytest.mark.parametrize("setup", [0, 1])
+@pytest.mark.parametrize("name", scene_names())
+def test_exact_matches_fp32_oracle_bitwise(gpu, name, setup):
+    _label(name, setup)
+    sname, cam, light, phong, cam6, l3, ph8 = setups()[setup]
+    w, h = 160, 90
+    text = scene_text(name)
+    g = gpu_render(text, cam, light, phong, w, h, C.MODE_EXACT, counters=True)
+    o = O.OracleScene(text).render(cam6, l3, ph8, w, h, bits=32, counters=True)
+    assert np.array_equal(g["prim_id"], o["prim_id"])
+    assert np.array_equal(g["flags"], o["flags"])
+    assert np.array_equal(g["depth"].view(np.uint32), o["depth"].view(np.uint32))
+    assert np.array_equal(g["counters"], o["counters"])
+    assert np.abs(g["rgba"] - o["rgba"]).max() <= RGBA_TOL
+
+
+def compare_fp64(g, o64, depth_tol=DEPTH_REL_TOL):
+    unflagged = o64["degen"] == 0
+    cls = C.ENTER | C.EXIT | C.MISS
+    agree = (g["prim_id"] == o64["prim_id"]) & ((g["flags"] & cls) == (o64["flags"] & cls))
+    n_unf = int(unflagged.sum())
+    rate = float((agree & unflagged).sum()) / max(n_unf, 1)
+    hit = agree & (o64["prim_id"] >= 0)
+    rel = np.abs(g["depth"][hit].astype(np.float64) - o64["depth"][hit]) / np.abs(o64["depth"][hit])
+    return rate, (float(rel.max()) if rel.size else 0.0), float(1 - unflagged.mean())
+
+
+@pytest.mark.parametrize("setup", [0, 1])
+@pytest.mark.parametrize("name", scene_names())
+def test_fast_vs_fp64_oracle(gpu, name, setup):
+    _label(name, setup)
+    sname, cam, light, phong, cam6, l3, ph8 = setups()[setup]
+    w, h = 320, 180
+    text = scene_text(name)
+    g = gpu_render(text, cam, light, phong, w, h, C.MODE_FAST)
+    o64 = O.OracleScene(text).render(cam6, l3, ph8, w, h, bits=64, degen=True)
+    rate, drel, flagged = compare_fp64(g, o64)
+    print(f"agree(unflagged)={rate:.6f} depth_rel_max={drel:.2e} flagged={flagged:.4%}")
+    assert rate >= AGREE_MIN
+    assert drel <= DEPTH_REL_TOL
+
+
+def test_exact_vs_fp64_wikipedia_full_size(gpu):
+    """The Wikipedia scene at 640x360 from (3, 2.5, 4): exact mode against FP64."""
+    _label("testWikipedia.txt 640x360", 1)
+    sname, cam, light, phong, cam6, l3, ph8 = setups()[1]
+    text = scene_text("testWikipedia.txt")
+    g = gpu_render(text, cam, light, phong, 640, 360, C.MODE_EXACT)
+    o32 = O.OracleScene(text).render(cam6, l3, ph8, 640, 360, bits=32)
+    assert np.array_equal(g["prim_id"], o32["prim_id"])
+    o64 = O.OracleScene(text).render(cam6, l3, ph8, 640, 360, bits=64, degen=True)
+    rate, drel, flagged = compare_fp64(g, o64)
+    print(f"agree(unflagged)={rate:.6f} depth_rel_max={drel:.2e} flagged={flagged:.4%}")
+    assert rate >= AGREE_MIN and drel <= DEPTH_REL_TOL
+
+
+def test_render_is_deterministic_and_stream_ordered(gpu):
+    sname, cam, light, phong, cam6, l3, ph8 = setups()[1]
+    s = C.Scene(scene_text("testCheese256.txt"))
+    a = s.render(cam, light, phong, 257, 131)
+    b = s.render(cam, light, phong, 257, 131)
+    torch.cuda.synchronize()
+    for k in a:
+        assert torch.equal(a[k], b[k]), k
+
+
+def test_null_outputs_allowed(gpu):
+    sname, cam, light, phong, cam6, l3, ph8 = setups()[0]
+    s = C.Scene(scene_text("test5.txt"))
+    full = s.render(cam, light, phong, 64, 48)
+    part = s.render(cam, light, phong, 64, 48, outputs={"prim_id": torch.empty((48, 64), dtype=torch.int32,
+                                                                                  device="cuda")})
+    torch.cuda.synchronize()
+    assert torch.equal(full["prim_id"], part["prim_id"])
+
+
+def test_render_argument_errors(gpu):
+    sname, cam, light, phong, *_ = setups()[0]
+    s = C.Scene(scene_text("test1.txt"))
+    for w, h in ((1, 10), (10, 1), (0, 0)):
+        with pytest.raises(C.CsgError) as e:
+            s.render(cam, light, phong, w, h)
+        assert e.value.status == C.E_ARG
+    host = C.Scene(scene_text("test1.txt"), upload=False)
+    with pytest.raises(C.CsgError):
+        host.render(cam, light, phong, 16, 16)
+    host.upload()
+    out = host.render(cam, light, phong, 16, 16)
+    torch.cuda.synchronize()
+    assert out["prim_id"].shape == (16, 16)
