@@ -80,6 +80,7 @@ def main():
     ph = C.phong(*S.PHONG)
     mode = C.MODE_FAST if args.mode == "fast" else C.MODE_EXACT
     scene = C.Scene(text)
+    plan = scene.plan(mode=mode)
     outs = {
         "rgba": torch.empty((h, w, 4), dtype=torch.float32, device=dev),
         "depth": torch.empty((h, w), dtype=torch.float32, device=dev),
@@ -136,7 +137,8 @@ def main():
         "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded scene)",
         "config": {"workload": "plain: cheese 512 spheres, 1920x1080, 1 ray/pixel", "mode": args.mode,
                    "rays_per_step": rays, "p50_ms": round(p50, 4), "p99_ms": round(p99, 4),
-                   "wall_s": round(wall, 3)},
+                   "wall_s": round(wall, 3), "block": plan.block, "grid": plan.grid,
+                   "scene_in_lds": bool(plan.scene_in_lds), "lds_bytes": plan.lds_bytes},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

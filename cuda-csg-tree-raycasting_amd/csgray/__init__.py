@@ -21,6 +21,7 @@ OK, E_PARSE, E_DEPTH, E_OOM, E_HIP, E_ARG, E_BUDGET = range(7)
 STATUS_NAMES = {0: "CSG_OK", 1: "CSG_E_PARSE", 2: "CSG_E_DEPTH", 3: "CSG_E_OOM", 4: "CSG_E_HIP",
                 5: "CSG_E_ARG", 6: "CSG_E_BUDGET"}
 MODE_EXACT, MODE_FAST = 0, 1
+PLACE_AUTO, PLACE_GLOBAL, PLACE_LDS = 0, 1, 2
 
 # hit flag bits (final record)
 ENTER, EXIT, MISS, FLIP, CAP_BOTTOM, CAP_TOP = 2, 4, 8, 16, 32, 64
@@ -52,8 +53,13 @@ class SceneInfo(ctypes.Structure):
 
 
 class RenderOpts(ctypes.Structure):
-    _fields_ = [("mode", ctypes.c_int32), ("block", ctypes.c_int32), ("rays_per_lane", ctypes.c_int32),
+    _fields_ = [("mode", ctypes.c_int32), ("block", ctypes.c_int32), ("scene_placement", ctypes.c_int32),
                 ("counters", ctypes.c_int32), ("counter_out", ctypes.c_void_p)]
+
+
+class LaunchInfo(ctypes.Structure):
+    _fields_ = [("block", ctypes.c_int32), ("grid", ctypes.c_int32), ("lds_bytes", ctypes.c_int64),
+                ("scene_in_lds", ctypes.c_int32), ("compute_units", ctypes.c_int32), ("lds_limit", ctypes.c_int64)]
 
 
 class CsgError(RuntimeError):
@@ -87,6 +93,7 @@ _SIGNATURES = [
                                      ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(Outputs),
                                      ctypes.POINTER(RenderOpts), _P]),
     ("csg_sync", ctypes.c_int, [_P]),
+    ("csg_render_plan", ctypes.c_int, [_P, ctypes.POINTER(RenderOpts), ctypes.POINTER(LaunchInfo)]),
     ("csg_last_error", ctypes.c_char_p, []),
     ("csg_version", ctypes.c_char_p, []),
 ]
@@ -212,8 +219,14 @@ class Scene:
         _check(lib().csg_scene_export(self.handle, links.ctypes.data, boxes.ctypes.data, prims.ctypes.data))
         return links, boxes, prims
 
+    def plan(self, mode=MODE_FAST, block=0, placement=0) -> LaunchInfo:
+        info = LaunchInfo()
+        opts = RenderOpts(mode, block, placement, 0, None)
+        _check(lib().csg_render_plan(self.handle, ctypes.byref(opts), ctypes.byref(info)))
+        return info
+
     def render(self, cam, light_, phong_, width, height, mode=MODE_FAST, outputs=None, counters=False,
-               stream=None, device=None):
+               stream=None, device=None, block=0, placement=0):
         """Render into torch CUDA tensors; returns dict rgba/depth/prim_id/flags(/counters).
 
         `outputs` may pass preallocated tensors (same keys) to avoid allocation."""
@@ -234,7 +247,7 @@ class Scene:
         assert outputs["rgba"].numel() >= 4 * n if "rgba" in outputs else True
         o = Outputs(*(outputs[k].data_ptr() if outputs.get(k) is not None else None
                       for k in ("rgba", "depth", "prim_id", "flags")))
-        opts = RenderOpts(mode, 0, 0, 1 if counters else 0,
+        opts = RenderOpts(mode, block, placement, 1 if counters else 0,
                           outputs["counters"].data_ptr() if counters else None)
         s = stream if stream is not None else torch.cuda.current_stream(dev).cuda_stream
         _check(lib().csg_render_ex(self.handle, ctypes.byref(cam), ctypes.byref(light_), ctypes.byref(phong_),

@@ -3,8 +3,11 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <map>
+#include <mutex>
 #include <cstring>
 #include <new>
 #include <string>
@@ -41,35 +44,65 @@ void CopyErr(char* err, size_t errlen, const std::string& msg) {
 
 size_t Align256(size_t n) { return (n + 255) & ~static_cast<size_t>(255); }
 
-// One device allocation holding the five scene arrays back to back.
+// One device allocation holding the scene arrays back to back.
 csg_status Upload(csg_scene* s) {
   const csgray::FlatScene& f = s->flat;
-  const size_t nb = Align256(sizeof(csgray::F4) * f.nNodes);
-  const size_t pb = Align256(sizeof(csgray::F4) * f.nPrims);
-  s->bytes = 2 * nb + 3 * pb;
+  const std::vector<csgray::F4>* src[4] = {&f.ops, &f.primA, &f.primB, &f.primC};
+  size_t off[4], total = 0;
+  for (int k = 0; k < 4; k++) {
+    off[k] = total;
+    total += Align256(sizeof(csgray::F4) * src[k]->size());
+  }
+  s->bytes = total;
   hipError_t e = hipGetDevice(&s->device);
   if (e != hipSuccess) return FailHip(e, "hipGetDevice");
-  e = hipMalloc(&s->dmem, s->bytes);
+  e = hipMalloc(&s->dmem, std::max<size_t>(s->bytes, 256));
   if (e != hipSuccess) {
     s->dmem = nullptr;
     return e == hipErrorOutOfMemory ? Fail(CSG_E_OOM, "hipMalloc of the scene failed") : FailHip(e, "hipMalloc");
   }
   char* base = static_cast<char*>(s->dmem);
-  char* parts[5] = {base, base + nb, base + 2 * nb, base + 2 * nb + pb, base + 2 * nb + 2 * pb};
-  const std::vector<csgray::F4>* src[5] = {&f.nodeLo, &f.nodeHi, &f.primA, &f.primB, &f.primC};
-  for (int k = 0; k < 5; k++) {
+  for (int k = 0; k < 4; k++) {
     if (src[k]->empty()) continue;
-    e = hipMemcpy(parts[k], src[k]->data(), sizeof(csgray::F4) * src[k]->size(), hipMemcpyHostToDevice);
+    e = hipMemcpy(base + off[k], src[k]->data(), sizeof(csgray::F4) * src[k]->size(), hipMemcpyHostToDevice);
     if (e != hipSuccess) return FailHip(e, "hipMemcpy of the scene");
   }
-  s->dev.nodeLo = reinterpret_cast<const float4*>(parts[0]);
-  s->dev.nodeHi = reinterpret_cast<const float4*>(parts[1]);
-  s->dev.primA = reinterpret_cast<const float4*>(parts[2]);
-  s->dev.primB = reinterpret_cast<const float4*>(parts[3]);
-  s->dev.primC = reinterpret_cast<const float4*>(parts[4]);
-  s->dev.nNodes = f.nNodes;
+  s->dev.ops = reinterpret_cast<const float4*>(base + off[0]);
+  s->dev.primA = reinterpret_cast<const float4*>(base + off[1]);
+  s->dev.primB = reinterpret_cast<const float4*>(base + off[2]);
+  s->dev.primC = reinterpret_cast<const float4*>(base + off[3]);
+  s->dev.rootDesc = f.rootDesc;
+  s->dev.nOps = f.nOps;
   s->dev.nPrims = f.nPrims;
   s->dev.opDepth = f.opDepth;
+  return CSG_OK;
+}
+
+// The render kernel's tile queue: two words per (device, stream), zero
+// between launches (the last wave of each launch zeroes them).  Renders on
+// one stream are ordered, so they can share it; separate streams get separate
+// queues.  Queues live until the process ends.
+std::mutex g_queueMutex;
+std::map<std::pair<int, hipStream_t>, uint32_t*> g_queues;
+
+csg_status QueueFor(int dev, hipStream_t stream, uint32_t** out) {
+  std::lock_guard<std::mutex> lock(g_queueMutex);
+  auto it = g_queues.find({dev, stream});
+  if (it != g_queues.end()) {
+    *out = it->second;
+    return CSG_OK;
+  }
+  void* q = nullptr;
+  hipError_t e = hipMalloc(&q, 256);
+  if (e != hipSuccess) return e == hipErrorOutOfMemory ? Fail(CSG_E_OOM, "queue allocation failed")
+                                                       : FailHip(e, "hipMalloc");
+  e = hipMemsetAsync(q, 0, 256, stream);
+  if (e != hipSuccess) {
+    (void)hipFree(q);
+    return FailHip(e, "hipMemsetAsync");
+  }
+  g_queues[{dev, stream}] = static_cast<uint32_t*>(q);
+  *out = static_cast<uint32_t*>(q);
   return CSG_OK;
 }
 
@@ -258,6 +291,44 @@ void csg_phong_default(csg_phong* ph) {  // RaycastingKernels.cu:71-74, 109
   ph->bg[0] = 0.08f; ph->bg[1] = 0.08f; ph->bg[2] = 0.11f; ph->bg[3] = 1.0f;
 }
 
+namespace {
+csg_status ParseOpts(const csg_render_opts* opts, csgray::LaunchConfig* cfg) {
+  if (!opts) return CSG_OK;
+  if (opts->mode != CSG_MODE_EXACT && opts->mode != CSG_MODE_FAST) return Fail(CSG_E_ARG, "csg_render: unknown mode");
+  cfg->mode = opts->mode == CSG_MODE_EXACT ? csgray::Mode::kExact : csgray::Mode::kFast;
+  cfg->count = opts->counters != 0;
+  if (cfg->count && !opts->counter_out) return Fail(CSG_E_ARG, "csg_render: counters requested without a buffer");
+  if (opts->block != 0 && opts->block != 256 && opts->block != 512 && opts->block != 1024)
+    return Fail(CSG_E_ARG, "csg_render: block must be 0, 256, 512 or 1024");
+  if (opts->scene_placement < 0 || opts->scene_placement > 2)
+    return Fail(CSG_E_ARG, "csg_render: scene_placement must be 0, 1 or 2");
+  cfg->block = opts->block;
+  cfg->sceneInLds = opts->scene_placement == 0 ? -1 : (opts->scene_placement == 2 ? 1 : 0);
+  return CSG_OK;
+}
+}  // namespace
+
+csg_status csg_render_plan(const csg_scene* s, const csg_render_opts* opts, csg_launch_info* info) {
+  if (!s || !info) return Fail(CSG_E_ARG, "csg_render_plan: NULL argument");
+  csgray::LaunchConfig cfg;
+  const csg_status st = ParseOpts(opts, &cfg);
+  if (st != CSG_OK) return st;
+  int cus = 0;
+  size_t ldsLimit = 0;
+  hipError_t e = csgray::DeviceShapeLimits(&cus, &ldsLimit);
+  if (e != hipSuccess) return FailHip(e, "device query");
+  csgray::LaunchShape shape;
+  e = csgray::ChooseShape(s->dev, cfg, &shape);
+  if (e != hipSuccess) return FailHip(e, "choosing the launch shape");
+  info->block = shape.block;
+  info->grid = shape.grid;
+  info->lds_bytes = static_cast<int64_t>(shape.lds);
+  info->scene_in_lds = shape.sceneInLds ? 1 : 0;
+  info->compute_units = cus;
+  info->lds_limit = static_cast<int64_t>(ldsLimit);
+  return CSG_OK;
+}
+
 csg_status csg_render_ex(const csg_scene* s, const csg_camera* cam, const csg_light* light, const csg_phong* phong,
                          int32_t width, int32_t height, const csg_outputs* out, const csg_render_opts* opts,
                          void* stream) {
@@ -268,13 +339,8 @@ csg_status csg_render_ex(const csg_scene* s, const csg_camera* cam, const csg_li
   if (static_cast<int64_t>(width) * height > (int64_t{1} << 31))
     return Fail(CSG_E_ARG, "csg_render: image larger than 2^31 pixels");
   csgray::LaunchConfig cfg;
-  if (opts) {
-    if (opts->mode != CSG_MODE_EXACT && opts->mode != CSG_MODE_FAST)
-      return Fail(CSG_E_ARG, "csg_render: unknown mode");
-    cfg.mode = opts->mode == CSG_MODE_EXACT ? csgray::Mode::kExact : csgray::Mode::kFast;
-    cfg.count = opts->counters != 0;
-    if (cfg.count && !opts->counter_out) return Fail(CSG_E_ARG, "csg_render: counters requested without a buffer");
-  }
+  const csg_status ost = ParseOpts(opts, &cfg);
+  if (ost != CSG_OK) return ost;
   if (!s->dmem) return Fail(CSG_E_ARG, "csg_render: scene is not on a device (call csg_scene_upload)");
   int dev = 0;
   hipError_t e = hipGetDevice(&dev);
@@ -301,7 +367,17 @@ csg_status csg_render_ex(const csg_scene* s, const csg_camera* cam, const csg_li
   p.primId = out->prim_id;
   p.flags = out->flags;
   p.counters = cfg.count ? opts->counter_out : nullptr;
-  e = csgray::LaunchRender(p, cfg, static_cast<hipStream_t>(stream));
+  p.tilesX = (width + 7) / 8;
+  const int64_t tiles = static_cast<int64_t>(p.tilesX) * ((height + 7) / 8);
+  p.numTiles = static_cast<int32_t>(tiles);
+  csgray::LaunchShape shape;
+  e = csgray::ChooseShape(s->dev, cfg, &shape);
+  if (e != hipSuccess) return FailHip(e, "choosing the launch shape");
+  p.totalWaves = static_cast<uint32_t>(shape.grid) * static_cast<uint32_t>(shape.block / 64);
+  const hipStream_t hs = static_cast<hipStream_t>(stream);
+  const csg_status qs = QueueFor(dev, hs, &p.queue);
+  if (qs != CSG_OK) return qs;
+  e = csgray::LaunchRender(p, cfg, shape, hs);
   if (e != hipSuccess) return FailHip(e, "render launch");
   return CSG_OK;
 }

@@ -6,16 +6,27 @@
 // :49-111, runs as a second launch over a 36-byte-per-pixel hit buffer; here
 // that buffer never exists).
 //
-// Traversal state per lane lives in registers except the stack.  The
-// reference keeps three stacks of 32 entries in local memory (action bytes,
-// 8-byte hit records, float times; CudaStack.cuh:5-44).  Their pushes and pops
-// nest per tree level: an operator on the current root-to-node path owns
-// either one saved hit record (with a LoadLft/LoadRgh action) or one saved
-// tmin (with a LoadLft+SaveLft pair), and SaveLft turns the saved tmin into a
-// saved hit in place.  So one 8-byte LDS slot per operator level holds
-// everything: {float t or tmin, uint32 hit bits | action code}.  The slot
-// count is the scene's operator depth, not 32, and the slab is laid out
-// [slot][lane] so a wave's accesses to any mix of slots are bank-conflict free.
+// Work distribution: a persistent grid; each wave takes 8x8-pixel tiles from
+// a global queue until none are left, so expensive tiles do not hold back a
+// whole workgroup.
+//
+// Scene access: the traversal only ever stands on operators.  Stepping down
+// into an operator reads its 64-byte record (both children's boxes and
+// descriptors); stepping back up reads the parent's 16-byte descriptor word;
+// a primitive child's parameters are read only when its box is hit.  When the
+// scene fits, every workgroup stages it in LDS once and all reads are LDS
+// reads; otherwise they go to global memory (L2-resident).
+//
+// Stack: the reference keeps three stacks of 32 entries in local memory
+// (action bytes, 8-byte hit records, float times; CudaStack.cuh:5-44).  Their
+// pushes and pops nest per tree level: an operator on the current
+// root-to-node path owns either one saved hit record (with a LoadLft/LoadRgh
+// action) or one saved tmin (with a LoadLft+SaveLft pair), and SaveLft turns
+// the saved tmin into a saved hit in place.  So one 8-byte LDS slot per
+// operator level holds everything: {float t or tmin, uint32 hit bits | action
+// code}.  The slot count is the scene's operator depth, and the slab is laid
+// out [slot][lane] so a wave's accesses to any mix of slots are bank-conflict
+// free.
 //
 // Arithmetic: Mode::kExact evaluates every expression as the reference writes
 // it, with IEEE division and square root and no contraction (the library is
@@ -25,6 +36,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 
 #include "device.hpp"
@@ -39,7 +51,8 @@ constexpr int kGotoLft = 2, kGotoRgh = 4, kCompute = 8, kLoadLft = 16, kLoadRgh 
 // Hit actions (CSGUtils.cuh:17-25).
 constexpr uint32_t kAMiss = 2, kRetL = 4, kRetR = 8, kLoopL = 16, kLoopR = 32, kLoopRIfCloser = 64,
                    kLoopLIfCloser = 128, kRetLIfCloser = 256, kRetRIfCloser = 512, kFlipR = 1024;
-constexpr int kSphereT = 3, kCylinderT = 4, kCubeT = 5;
+constexpr uint32_t kSphereT = 3, kCylinderT = 4, kCubeT = 5;
+constexpr uint32_t kNone = 0xFFFFFFFFu;
 
 // Action tables (RaycastingKernels.cu:666-677), index op*9 + l*3 + r with
 // states Enter 0, Exit 1, Miss 2 and ops Union 0, Difference 1, Intersection 2.
@@ -67,16 +80,6 @@ struct Hit {
   uint32_t bits;
 };
 
-__device__ __forceinline__ Hit MissRecord() { return Hit{-1.0f, kMissBit | 0xFFFF0000u}; }
-__device__ __forceinline__ uint32_t Flags(uint32_t bits) { return bits & 0xFFu; }
-__device__ __forceinline__ uint32_t WithFlags(uint32_t bits, uint32_t f) { return (bits & ~0xFFu) | f; }
-__device__ __forceinline__ uint32_t State(uint32_t bits) {  // LookUpActions, :679-691
-  return (bits & kEnter) ? 0u : ((bits & kExit) ? 1u : 2u);
-}
-__device__ __forceinline__ uint32_t HitBits(uint32_t flags, int type, int prim) {
-  return flags | (static_cast<uint32_t>(type) << 8) | (static_cast<uint32_t>(prim) << 16);
-}
-
 // Flag bit (unused by the reference) marking a ray whose traversal was cut
 // short by the stack bound or the iteration cap; it never appears for a
 // well-formed tree and the tests require it absent.
@@ -88,7 +91,18 @@ constexpr int kIterCap = 1 << 20;
 constexpr uint32_t kSlotLoadLft = 1u << 11, kSlotLoadRgh = 2u << 11, kSlotSaveLft = 3u << 11,
                    kSlotMask = 3u << 11;
 
+__device__ __forceinline__ Hit MissRecord() { return Hit{-1.0f, kMissBit | 0xFFFF0000u}; }
+__device__ __forceinline__ uint32_t Flags(uint32_t bits) { return bits & 0xFFu; }
+__device__ __forceinline__ uint32_t WithFlags(uint32_t bits, uint32_t f) { return (bits & ~0xFFu) | f; }
+__device__ __forceinline__ uint32_t State(uint32_t bits) {  // LookUpActions, :679-691
+  return (bits & kEnter) ? 0u : ((bits & kExit) ? 1u : 2u);
+}
+__device__ __forceinline__ uint32_t HitBits(uint32_t flags, uint32_t type, uint32_t prim) {
+  return flags | (type << 8) | (prim << 16);
+}
 __device__ __forceinline__ uint32_t AsU(float f) { return __float_as_uint(f); }
+__device__ __forceinline__ uint32_t DescType(uint32_t d) { return d & 0xFFu; }
+__device__ __forceinline__ uint32_t DescIndex(uint32_t d) { return d >> 8; }
 
 struct RayF {
   float ox, oy, oz;
@@ -129,9 +143,10 @@ __device__ __forceinline__ void Slab(const RayF& r, float lx, float ly, float lz
 
 // isBVHNodeHit (:718-757).  The caller turns a miss into hit = Miss.
 template <bool kFast>
-__device__ __forceinline__ bool BoxHit(const RayF& r, const float4& lo, const float4& hi, float tmin) {
+__device__ __forceinline__ bool BoxHit(const RayF& r, float lx, float ly, float lz, float hx, float hy, float hz,
+                                       float tmin) {
   float tmn, tmx;
-  Slab<kFast>(r, lo.x, lo.y, lo.z, hi.x, hi.y, hi.z, tmn, tmx);
+  Slab<kFast>(r, lx, ly, lz, hx, hy, hz, tmn, tmx);
   if (tmx < 0) return false;
   if (tmn > tmx) return false;
   if (tmn <= tmin && tmx <= tmin) return false;
@@ -139,34 +154,30 @@ __device__ __forceinline__ bool BoxHit(const RayF& r, const float4& lo, const fl
 }
 
 // sphereHit (:135-181).
-__device__ __forceinline__ void SphereHit(const RayF& r, const float4& a, int prim, Hit& h, float tmin) {
+__device__ __forceinline__ Hit SphereHit(const RayF& r, const float4& a, uint32_t prim, float tmin, Hit h) {
   h.bits = HitBits(kMissBit, 0, prim);
   const float ocx = r.ox - a.x, ocy = r.oy - a.y, ocz = r.oz - a.z;
   const float b = ocx * r.dx + ocy * r.dy + ocz * r.dz;
   const float c = (ocx * ocx + ocy * ocy + ocz * ocz) - a.w * a.w;
   const float disc = b * b - c;
-  if (disc < 0) return;
-  float temp = (-b - sqrtf(disc));
+  if (disc < 0) return h;
+  const float sq = sqrtf(disc);
+  float temp = (-b - sq);
   if (temp <= tmin) {
-    temp = (-b + sqrtf(disc));
-    if (temp <= tmin) {
-      h.t = -1.0f;
-      h.bits = HitBits(kMissBit, 0, 0xFFFF);
-      return;
-    }
+    temp = (-b + sq);
+    if (temp <= tmin) return Hit{-1.0f, HitBits(kMissBit, 0, 0xFFFFu)};
   }
-  h.t = temp;
   const float nx = (r.ox + temp * r.dx) - a.x;
   const float ny = (r.oy + temp * r.dy) - a.y;
   const float nz = (r.oz + temp * r.dz) - a.z;
   const float dn = nx * r.dx + ny * r.dy + nz * r.dz;
-  h.bits = HitBits(dn <= 0 ? kEnter : kExit, kSphereT, prim);
+  return Hit{temp, HitBits(dn <= 0 ? kEnter : kExit, kSphereT, prim)};
 }
 
 // cylinderHit (:202-336).  Both roots go through the same cap logic, as in
 // the reference's duplicated blocks (:238-266, :273-301).
-__device__ __forceinline__ void CylinderHit(const RayF& r, const float4& a, const float4& bx, int prim, Hit& h,
-                                            float tmin) {
+__device__ __forceinline__ Hit CylinderHit(const RayF& r, const float4& a, const float4& bx, uint32_t prim,
+                                           float tmin, Hit h) {
   h.bits = HitBits(kMissBit, 0, prim);
   const float vx = bx.x, vy = bx.y, vz = bx.z, height = bx.w, radius = a.w;
   const float hh = height / 2;
@@ -179,10 +190,11 @@ __device__ __forceinline__ void CylinderHit(const RayF& r, const float4& a, cons
   const float dOC = r.dx * OCx + r.dy * OCy + r.dz * OCz;
   const float b = (dOC - dV * ocV);
   const float disc = b * b - A * c;
-  if (disc < 0) return;
-  const float t1 = (-b - sqrtf(disc)) / (A), t2 = (-b + sqrtf(disc)) / (A);
+  if (disc < 0) return h;
+  const float sq = sqrtf(disc);
+  const float t1 = (-b - sq) / (A), t2 = (-b + sq) / (A);
   const float m1 = t1 * dV + ocV, m2 = t2 * dV + ocV;
-  if ((m1 < 0 && m2 < 0) || (m1 > height && m2 > height)) return;
+  if ((m1 < 0 && m2 < 0) || (m1 > height && m2 > height)) return h;
 
   // dot(d, -V), dot(-OC, -V) and dot(-OCmMax, V) written out as the
   // reference evaluates them.
@@ -193,7 +205,7 @@ __device__ __forceinline__ void CylinderHit(const RayF& r, const float4& a, cons
 
   float temp = t1, m = m1;
   bool skip = false;
-  int cap = 0;
+  uint32_t cap = 0;
   for (int root = 0; root < 2; root++) {
     if (root == 1) {
       temp = t2;
@@ -218,9 +230,8 @@ __device__ __forceinline__ void CylinderHit(const RayF& r, const float4& a, cons
       }
     }
     if (!(temp <= tmin || skip)) break;
-    if (root == 1) return;
+    if (root == 1) return h;
   }
-  h.t = temp;
   float nx, ny, nz;
   if (cap == 1) {
     nx = -vx; ny = -vy; nz = -vz;
@@ -235,71 +246,81 @@ __device__ __forceinline__ void CylinderHit(const RayF& r, const float4& a, cons
   uint32_t f = dn <= 0 ? kEnter : kExit;
   if (cap == 1) f |= kFlag1;
   else if (cap == 2) f |= kFlag2;
-  h.bits = HitBits(f, kCylinderT, prim);
+  return Hit{temp, HitBits(f, kCylinderT, prim)};
 }
 
 // cubeHit (:375-434).
 template <bool kFast>
-__device__ __forceinline__ void CubeHit(const RayF& r, const float4& a, int prim, Hit& h, float tmin) {
+__device__ __forceinline__ Hit CubeHit(const RayF& r, const float4& a, uint32_t prim, float tmin, Hit h) {
   h.bits = HitBits(kMissBit, 0, prim);
   const float half = a.w / 2;
   const float lx = a.x - half * 1.0f, ly = a.y - half * 1.0f, lz = a.z - half * 1.0f;
   const float hx = a.x + half * 1.0f, hy = a.y + half * 1.0f, hz = a.z + half * 1.0f;
   float tmn, tmx;
   Slab<kFast>(r, lx, ly, lz, hx, hy, hz, tmn, tmx);
-  if (tmx < 0) return;
-  if (tmn > tmx) return;
+  if (tmx < 0) return h;
+  if (tmn > tmx) return h;
   if (tmn <= tmin) {
     tmn = tmx;
-    if (tmn <= tmin) return;
+    if (tmn <= tmin) return h;
   }
-  h.t = tmn;
   const float px = (r.ox + tmn * r.dx) - a.x;
   const float py = (r.oy + tmn * r.dy) - a.y;
   const float pz = (r.oz + tmn * r.dz) - a.z;
   const float bias = 1.00001f;
   const float nx = TruncInt(px / half * bias), ny = TruncInt(py / half * bias), nz = TruncInt(pz / half * bias);
   const float dn = nx * r.dx + ny * r.dy + nz * r.dz;
-  h.bits = HitBits(dn <= 0 ? kEnter : kExit, kCubeT, prim);
+  return Hit{tmn, HitBits(dn <= 0 ? kEnter : kExit, kCubeT, prim)};
 }
 
 struct Counts {
   int32_t go, comp, box, sph, cyl, cub;
 };
 
-// hitPrimitive (:113-133) for a node with the given meta word.
+// Scene arrays, either LDS copies or the global originals.
+struct SceneView {
+  const float4* ops;
+  const float4* primA;
+  const float4* primB;
+};
+
+// hitPrimitive (:113-133) for a primitive descriptor.  `h` is the record the
+// result replaces (its stale fields survive a miss, as in the reference).
 template <bool kFast, bool kCount>
-__device__ __forceinline__ void HitPrimitive(const DevScene& sc, const RayF& r, uint32_t meta, Hit& h, float tmin,
-                                             Counts& n) {
-  const int type = static_cast<int>(meta & 0xFFu);
-  const int prim = static_cast<int>(meta >> 8);
+__device__ __forceinline__ Hit Intersect(const SceneView& sv, const RayF& r, uint32_t desc, float tmin, Hit h,
+                                         Counts& n) {
+  const uint32_t type = DescType(desc), prim = DescIndex(desc);
+  const float4 a = sv.primA[prim];
   if (type == kSphereT) {
     if (kCount) n.sph++;
-    SphereHit(r, sc.primA[prim], prim, h, tmin);
-  } else if (type == kCylinderT) {
-    if (kCount) n.cyl++;
-    CylinderHit(r, sc.primA[prim], sc.primB[prim], prim, h, tmin);
-  } else if (type == kCubeT) {
-    if (kCount) n.cub++;
-    CubeHit<kFast>(r, sc.primA[prim], prim, h, tmin);
-  } else {
-    h = MissRecord();
+    return SphereHit(r, a, prim, tmin, h);
   }
+  if (type == kCylinderT) {
+    if (kCount) n.cyl++;
+    return CylinderHit(r, a, sv.primB[prim], prim, tmin, h);
+  }
+  if (kCount) n.cub++;
+  return CubeHit<kFast>(r, a, prim, tmin, h);
 }
 
 // CSGRayCast (:459-512) with GoTo (:514-595), Compute (:597-661) and
 // GetParent (:708-716).  `stk` points at this lane's slot 0; slot s is at
 // stk[s * stride].
+//
+// The step is written for SIMT execution: decisions are selects rather than
+// nested branches, stack writes and reads are unconditional within their
+// step (indices clamped to the slab, an out-of-range access marks the ray
+// aborted instead of branching), and the primitive tests of a step run
+// through one call site.  Lanes of a wave sit at different points of their
+// own state machine, so every branch a lane takes costs the whole wave.
 template <bool kFast, bool kCount>
-__device__ Hit Trace(const DevScene& sc, const RayF& r, uint2* stk, int stride, int cap, const uint16_t* table,
-                     Counts& n) {
-  const uint32_t rootMeta = AsU(sc.nodeLo[0].w);
-  if ((rootMeta & 0xFFu) >= static_cast<uint32_t>(kSphereT)) {
+__device__ Hit Trace(const SceneView& sv, const RayF& r, uint32_t rootDesc, uint2* stk, int stride, int cap,
+                     const uint16_t* table, Counts& n) {
+  if (DescType(rootDesc) >= kSphereT) {
     // A primitive root: GoTo intersects it with tmin 0 and the state machine
     // ends without changing the record's classification.
-    Hit h = MissRecord();
     if (kCount) n.go++;
-    HitPrimitive<kFast, kCount>(sc, r, rootMeta, h, 0.0f, n);
+    Hit h = Intersect<kFast, kCount>(sv, r, rootDesc, 0.0f, MissRecord(), n);
     if (kCount) n.comp++;
     if (Flags(h.bits) & kMissBit) h = MissRecord();
     return h;
@@ -307,128 +328,126 @@ __device__ Hit Trace(const DevScene& sc, const RayF& r, uint2* stk, int stride, 
 
   Hit L = MissRecord(), R = MissRecord(), held = MissRecord();
   float tmin = 0.0f;
-  int cur = -1;  // the reference's virtual parent of the root; its left child is node 0
-  uint32_t curMeta = 0, curLink = 0;
+  // The operator the machine stands on.  Initially the reference's virtual
+  // parent of the root, whose left child is the root.
+  uint32_t opType = 0, opParent = kNone, lDesc = rootDesc, rDesc = 0;
   int action = kGotoLft;
   int sp = 0;
-  bool done = false;
-
-  auto pop = [&]() {
-    sp--;
-    const uint2 e = stk[sp * stride];
-    held.t = __uint_as_float(e.x);
-    held.bits = e.y & ~kSlotMask;
-    const uint32_t code = e.y & kSlotMask;
-    action = code == kSlotLoadLft ? kLoadLft : (code == kSlotLoadRgh ? kLoadRgh : kSaveLft);
-  };
-  bool aborted = false;
-  auto push = [&](float t, uint32_t bits, uint32_t code) {
-    if (sp >= cap) {
-      aborted = true;
-      done = true;
-      return;
-    }
-    stk[sp * stride] = make_uint2(AsU(t), (bits & ~kSlotMask) | code);
-    sp++;
-  };
-  auto moveTo = [&](int idx) {
-    cur = idx;
-    curMeta = AsU(sc.nodeLo[idx].w);
-    curLink = AsU(sc.nodeHi[idx].w);
-  };
-  auto ret = [&]() {
-    const uint32_t parent = curLink >> 16;
-    if (parent == 0xFFFFu || sp <= 0) {
-      done = true;
-    } else {
-      pop();
-      moveTo(static_cast<int>(parent));
-    }
-  };
-
+  bool done = false, aborted = false;
   int iter = 0;
+
   while (!done) {
-    if (++iter > kIterCap) {
-      aborted = true;
-      break;
-    }
     if (action == kSaveLft) {
+      // tmin saved by the GoTo that descended left comes back, and the left
+      // result takes its slot (the slot was popped by the Compute that
+      // returned here).
       tmin = held.t;
-      push(L.t, L.bits, kSlotLoadLft);
+      stk[sp * stride] = make_uint2(AsU(L.t), (L.bits & ~kSlotMask) | kSlotLoadLft);
+      sp++;
       action = kGotoRgh;
     }
-    if (action & (kGotoLft | kGotoRgh)) {
+    if (action == kGotoLft || action == kGotoRgh) {
       if (kCount) n.go++;
-      moveTo(action == kGotoLft ? cur + 1 : static_cast<int>(curLink & 0xFFFFu));
-      if ((curMeta & 0xFFu) < static_cast<uint32_t>(kSphereT)) {
-        const int li = cur + 1, ri = static_cast<int>(curLink & 0xFFFFu);
-        const float4 llo = sc.nodeLo[li], lhi = sc.nodeHi[li];
-        const float4 rlo = sc.nodeLo[ri], rhi = sc.nodeHi[ri];
-        bool gL = BoxHit<kFast>(r, llo, lhi, tmin);
-        bool gR = BoxHit<kFast>(r, rlo, rhi, tmin);
+      const bool left = action == kGotoLft;
+      const uint32_t d = left ? lDesc : rDesc;
+      if (DescType(d) < kSphereT) {
+        const uint32_t k = DescIndex(d);
+        const float4 q0 = sv.ops[4 * k + 0];
+        const float4 q1 = sv.ops[4 * k + 1];
+        const float4 q2 = sv.ops[4 * k + 2];
+        const float4 q3 = sv.ops[4 * k + 3];
+        lDesc = AsU(q3.x);
+        rDesc = AsU(q3.y);
+        opType = AsU(q3.z);
+        opParent = AsU(q3.w);
+        const bool gL = BoxHit<kFast>(r, q0.x, q0.y, q0.z, q1.x, q1.y, q1.z, tmin);
+        const bool gR = BoxHit<kFast>(r, q0.w, q1.w, q2.x, q2.y, q2.z, q2.w, tmin);
         if (kCount) n.box += 2;
-        if (!gL) L.bits = WithFlags(L.bits, kMissBit);
-        if (!gR) R.bits = WithFlags(R.bits, kMissBit);
-        const uint32_t lm = AsU(llo.w), rm = AsU(rlo.w);
-        if (gL && (lm & 0xFFu) >= static_cast<uint32_t>(kSphereT)) {
-          HitPrimitive<kFast, kCount>(sc, r, lm, L, tmin, n);
-          gL = false;
-        }
-        if (gR && (rm & 0xFFu) >= static_cast<uint32_t>(kSphereT)) {
-          HitPrimitive<kFast, kCount>(sc, r, rm, R, tmin, n);
-          gR = false;
-        }
-        if (gL || gR) {
-          if (!gL) {
-            push(L.t, L.bits, kSlotLoadLft);
-            action = kGotoRgh;
-          } else if (!gR) {
-            push(R.t, R.bits, kSlotLoadRgh);
-            action = kGotoLft;
-          } else {
-            push(tmin, 0u, kSlotSaveLft);
-            action = kGotoLft;
+        L.bits = gL ? L.bits : WithFlags(L.bits, kMissBit);
+        R.bits = gR ? R.bits : WithFlags(R.bits, kMissBit);
+        const bool pL = gL && DescType(lDesc) >= kSphereT;
+        const bool pR = gR && DescType(rDesc) >= kSphereT;
+        if (pL || pR) {
+#pragma nounroll
+          for (int j = 0; j < 2; j++) {
+            if (j == 0 ? pL : pR) {
+              const Hit h = Intersect<kFast, kCount>(sv, r, j == 0 ? lDesc : rDesc, tmin, j == 0 ? L : R, n);
+              if (j == 0) L = h;
+              else R = h;
+            }
           }
-        } else {
-          action = kCompute;
         }
+        const bool dL = gL && !pL, dR = gR && !pR;
+        // both -> park tmin (LoadLft+SaveLft); right only -> park L; left only -> park R
+        const bool both = dL && dR;
+        const float pt = both ? tmin : (dR ? L.t : R.t);
+        const uint32_t pb = both ? kSlotSaveLft
+                                 : (dR ? ((L.bits & ~kSlotMask) | kSlotLoadLft) : ((R.bits & ~kSlotMask) | kSlotLoadRgh));
+        if (dL || dR) {
+          aborted |= sp >= cap;
+          stk[min(sp, cap - 1) * stride] = make_uint2(AsU(pt), pb);
+          sp++;
+        }
+        action = dL ? kGotoLft : (dR ? kGotoRgh : kCompute);
       } else {
-        // A primitive reached by a loop: intersect it beyond tmin, then
-        // hand the record to the parent's Compute.
-        if (action == kGotoLft) HitPrimitive<kFast, kCount>(sc, r, curMeta, L, tmin, n);
-        else HitPrimitive<kFast, kCount>(sc, r, curMeta, R, tmin, n);
-        ret();
+        // A primitive child re-entered by a loop: intersect beyond tmin and
+        // hand the record to this operator's Compute (its loop left a Load
+        // on the stack).
+        const Hit h = Intersect<kFast, kCount>(sv, r, d, tmin, left ? L : R, n);
+        if (left) L = h;
+        else R = h;
+        aborted |= sp <= 0;
+        sp = max(sp - 1, 0);
+        const uint2 e = stk[sp * stride];
+        held = Hit{__uint_as_float(e.x), e.y & ~kSlotMask};
+        const uint32_t code = e.y & kSlotMask;
+        action = code == kSlotLoadLft ? kLoadLft : (code == kSlotLoadRgh ? kLoadRgh : kSaveLft);
       }
     }
-    if (!done && (action & (kLoadLft | kLoadRgh | kCompute))) {
+    if (action == kCompute || action == kLoadLft || action == kLoadRgh) {
       if (kCount) n.comp++;
       if (action == kLoadLft) L = held;
-      else if (action == kLoadRgh) R = held;
-      const uint32_t op = curMeta & 0xFFu;
-      const uint32_t acts = table[op * 9u + State(L.bits) * 3u + State(R.bits)];
-      if ((acts & kRetL) || ((acts & kRetLIfCloser) && (L.t < R.t))) {
-        R = L;
-        ret();
-      } else if ((acts & kRetR) || ((acts & kRetRIfCloser) && (L.t > R.t))) {
-        if (acts & kFlipR) R.bits ^= (kFlip | kExit | kEnter);
-        L = R;
-        ret();
-      } else if ((acts & kLoopL) || ((acts & kLoopLIfCloser) && (L.t < R.t))) {
-        tmin = L.t;
-        push(R.t, R.bits, kSlotLoadRgh);
-        action = kGotoLft;
-      } else if ((acts & kLoopR) || ((acts & kLoopRIfCloser) && (L.t > R.t))) {
-        tmin = R.t;
-        push(L.t, L.bits, kSlotLoadLft);
-        action = kGotoRgh;
+      if (action == kLoadRgh) R = held;
+      const uint32_t acts = table[opType * 9u + State(L.bits) * 3u + State(R.bits)];
+      const bool lc = L.t < R.t, rc = L.t > R.t;
+      const bool retL = (acts & kRetL) || ((acts & kRetLIfCloser) && lc);
+      const bool retR = !retL && ((acts & kRetR) || ((acts & kRetRIfCloser) && rc));
+      const bool loopL = !retL && !retR && ((acts & kLoopL) || ((acts & kLoopLIfCloser) && lc));
+      const bool loopR = !retL && !retR && !loopL && ((acts & kLoopR) || ((acts & kLoopRIfCloser) && rc));
+      if (loopL || loopR) {
+        tmin = loopL ? L.t : R.t;
+        const Hit park = loopL ? R : L;
+        aborted |= sp >= cap;
+        stk[min(sp, cap - 1) * stride] =
+            make_uint2(AsU(park.t), (park.bits & ~kSlotMask) | (loopL ? kSlotLoadRgh : kSlotLoadLft));
+        sp++;
+        action = loopL ? kGotoLft : kGotoRgh;
       } else {
-        R = MissRecord();
-        L = MissRecord();
-        ret();
+        const uint32_t rb = (acts & kFlipR) ? (R.bits ^ (kFlip | kExit | kEnter)) : R.bits;
+        const Hit res = retL ? L : (retR ? Hit{R.t, rb} : MissRecord());
+        L = res;
+        R = res;
+        if (opParent == kNone) {
+          done = true;
+        } else {
+          const float4 q3 = sv.ops[4 * opParent + 3];
+          aborted |= sp <= 0;
+          sp = max(sp - 1, 0);
+          const uint2 e = stk[sp * stride];
+          held = Hit{__uint_as_float(e.x), e.y & ~kSlotMask};
+          const uint32_t code = e.y & kSlotMask;
+          action = code == kSlotLoadLft ? kLoadLft : (code == kSlotLoadRgh ? kLoadRgh : kSaveLft);
+          lDesc = AsU(q3.x);
+          rDesc = AsU(q3.y);
+          opType = AsU(q3.z);
+          opParent = AsU(q3.w);
+        }
       }
     }
+    iter++;
+    done = done || aborted || iter >= kIterCap;
   }
-  if (aborted) L = Hit{-1.0f, kMissBit | kAborted | 0xFFFF0000u};
+  if (aborted || iter >= kIterCap) L = Hit{-1.0f, kMissBit | kAborted | 0xFFFF0000u};
   return L;
 }
 
@@ -439,21 +458,8 @@ __device__ __forceinline__ void Normalize3(float& x, float& y, float& z) {  // F
   z = inv * z;
 }
 
-template <bool kFast, bool kCount>
-__global__ void __launch_bounds__(256) RenderKernel(RenderParams p) {
-  extern __shared__ uint2 stackSlab[];
-  __shared__ uint16_t table[27];
-  const int tid = threadIdx.x;
-  if (tid < 27) table[tid] = kActionTable[tid];
-  __syncthreads();
-
-  // 16x16 pixel tile per 256-lane workgroup, one 8x8 tile per wave.
-  const int wave = tid >> 6, lane = tid & 63;
-  const int x = blockIdx.x * 16 + (wave & 1) * 8 + (lane & 7);
-  const int y = blockIdx.y * 16 + (wave >> 1) * 8 + (lane >> 3);
-  if (x >= p.width || y >= p.height) return;
-
-  // Ray generation (RaycastingKernels.cu:11-27, Ray.cuh:12-18).
+// Ray for pixel (x, y) (RaycastingKernels.cu:11-27, Ray.cuh:12-18).
+__device__ __forceinline__ RayF MakeRay(const RenderParams& p, int x, int y) {
   const float width = static_cast<float>(p.width), height = static_cast<float>(p.height);
   const float u = (static_cast<float>(x) + 0.5f) / (width - 1);
   const float v = (static_cast<float>(y) + 0.5f) / (height - 1);
@@ -463,38 +469,42 @@ __global__ void __launch_bounds__(256) RenderKernel(RenderParams p) {
   float dy = p.right[1] * nx + p.up[1] * ny + p.forward[1];
   float dz = p.right[2] * nx + p.up[2] * ny + p.forward[2];
   Normalize3(dx, dy, dz);
-  {
-    const float invLen = 1.0f / sqrtf(dx * dx + dy * dy + dz * dz);
-    dx *= invLen;
-    dy *= invLen;
-    dz *= invLen;
-  }
+  const float invLen = 1.0f / sqrtf(dx * dx + dy * dy + dz * dz);
   RayF r;
-  r.ox = p.pos[0]; r.oy = p.pos[1]; r.oz = p.pos[2];
-  r.dx = dx; r.dy = dy; r.dz = dz;
-  r.ix = 1.0f / dx; r.iy = 1.0f / dy; r.iz = 1.0f / dz;
+  r.ox = p.pos[0];
+  r.oy = p.pos[1];
+  r.oz = p.pos[2];
+  r.dx = dx * invLen;
+  r.dy = dy * invLen;
+  r.dz = dz * invLen;
+  r.ix = 1.0f / r.dx;
+  r.iy = 1.0f / r.dy;
+  r.iz = 1.0f / r.dz;
+  return r;
+}
 
-  Counts n{0, 0, 0, 0, 0, 0};
-  const int cap = p.sc.opDepth > 0 ? p.sc.opDepth : 1;
-  const Hit h = Trace<kFast, kCount>(p.sc, r, stackSlab + tid, blockDim.x, cap, table, n);
-
-  const size_t pix = static_cast<size_t>(y) * p.width + x;
+// Hit details (:183-200, :338-372, :436-457) and Phong (LightningKernel
+// :62-110), writing the pixel's outputs.
+__device__ __forceinline__ void ShadeAndStore(const RenderParams& p, const SceneView& sv, const RayF& r, Hit h,
+                                              size_t pix) {
   const uint32_t flags = Flags(h.bits);
   float4 colour = make_float4(p.bg[0], p.bg[1], p.bg[2], p.bg[3]);
   float depth = __builtin_inff();
   int32_t id = -1;
   if (!(flags & kMissBit)) {
-    const int prim = static_cast<int>(h.bits >> 16);
-    const int type = static_cast<int>((h.bits >> 8) & 7u);
-    const float4 a = p.sc.primA[prim];
+    const uint32_t prim = h.bits >> 16;
+    const uint32_t type = (h.bits >> 8) & 7u;
+    const float4 a = sv.primA[prim];
     const float t = h.t;
     const float px = r.ox + t * r.dx, py = r.oy + t * r.dy, pz = r.oz + t * r.dz;
     float Nx, Ny, Nz;
-    if (type == kSphereT) {  // sphereHitDetails :183-200
-      Nx = px - a.x; Ny = py - a.y; Nz = pz - a.z;
+    if (type == kSphereT) {
+      Nx = px - a.x;
+      Ny = py - a.y;
+      Nz = pz - a.z;
       Normalize3(Nx, Ny, Nz);
-    } else if (type == kCylinderT) {  // cylinderHitDetails :338-372
-      const float4 b = p.sc.primB[prim];
+    } else if (type == kCylinderT) {
+      const float4 b = sv.primB[prim];
       if (flags & kFlag1) {
         Nx = -b.x; Ny = -b.y; Nz = -b.z;
       } else if (flags & kFlag2) {
@@ -504,10 +514,12 @@ __global__ void __launch_bounds__(256) RenderKernel(RenderParams p) {
         const float Cx = a.x - hh * b.x, Cy = a.y - hh * b.y, Cz = a.z - hh * b.z;
         const float OCx = r.ox - Cx, OCy = r.oy - Cy, OCz = r.oz - Cz;
         const float m = (r.dx * b.x + r.dy * b.y + r.dz * b.z) * t + (OCx * b.x + OCy * b.y + OCz * b.z);
-        Nx = (px - Cx) - m * b.x; Ny = (py - Cy) - m * b.y; Nz = (pz - Cz) - m * b.z;
+        Nx = (px - Cx) - m * b.x;
+        Ny = (py - Cy) - m * b.y;
+        Nz = (pz - Cz) - m * b.z;
         Normalize3(Nx, Ny, Nz);
       }
-    } else {  // cubeHitDetails :436-457
+    } else {
       const float half = a.w / 2;
       const float bias = 1.00001f;
       Nx = TruncInt((px - a.x) / half * bias);
@@ -518,7 +530,6 @@ __global__ void __launch_bounds__(256) RenderKernel(RenderParams p) {
     if (flags & kFlip) { Nx = -Nx; Ny = -Ny; Nz = -Nz; }
     if (flags & kExit) { Nx = -Nx; Ny = -Ny; Nz = -Nz; }
 
-    // Phong (LightningKernel :62-105), light colour white.
     const float4 col = p.sc.primC[prim];
     const float Lx = p.light[0], Ly = p.light[1], Lz = p.light[2];
     float Vx = p.pos[0] - px, Vy = p.pos[1] - py, Vz = p.pos[2] - pz;
@@ -540,35 +551,169 @@ __global__ void __launch_bounds__(256) RenderKernel(RenderParams p) {
     colour = make_float4(fminf(fmaxf(fr, 0.0f), 1.0f), fminf(fmaxf(fg, 0.0f), 1.0f), fminf(fmaxf(fb, 0.0f), 1.0f),
                          1.0f);
     depth = t;
-    id = prim;
+    id = static_cast<int32_t>(prim);
   }
   if (p.rgba) p.rgba[pix] = colour;
   if (p.depth) p.depth[pix] = depth;
   if (p.primId) p.primId[pix] = id;
   if (p.flags) p.flags[pix] = static_cast<uint8_t>(flags);
-  if (kCount && p.counters) {
-    int32_t* c = p.counters + 6 * pix;
-    c[0] = n.go; c[1] = n.comp; c[2] = n.box; c[3] = n.sph; c[4] = n.cyl; c[5] = n.cub;
+}
+
+// LDS: [stack slab: opDepth x blockDim x 8 B][scene copy when kLds].
+template <bool kFast, bool kCount, bool kLds>
+__global__ void __launch_bounds__(1024) RenderKernel(RenderParams p) {
+  extern __shared__ uint4 ldsBase[];
+  __shared__ uint16_t table[27];
+  const int tid = threadIdx.x;
+  const int cap = p.sc.opDepth > 0 ? p.sc.opDepth : 1;
+  uint2* slab = reinterpret_cast<uint2*>(ldsBase);
+  if (tid < 27) table[tid] = kActionTable[tid];
+
+  SceneView sv{p.sc.ops, p.sc.primA, p.sc.primB};
+  if (kLds) {
+    float4* lds = reinterpret_cast<float4*>(slab + static_cast<size_t>(cap) * blockDim.x);
+    const int nOps4 = 4 * p.sc.nOps, nP = p.sc.nPrims;
+    for (int i = tid; i < nOps4; i += blockDim.x) lds[i] = p.sc.ops[i];
+    for (int i = tid; i < nP; i += blockDim.x) {
+      lds[nOps4 + i] = p.sc.primA[i];
+      lds[nOps4 + nP + i] = p.sc.primB[i];
+    }
+    sv.ops = lds;
+    sv.primA = lds + nOps4;
+    sv.primB = lds + nOps4 + nP;
+  }
+  __syncthreads();
+
+  const int lane = tid & 63;
+  for (;;) {
+    uint32_t tile = 0;
+    if (lane == 0) tile = atomicAdd(p.queue, 1u);
+    tile = __builtin_amdgcn_readfirstlane(tile);
+    if (tile >= static_cast<uint32_t>(p.numTiles)) break;
+    const int x = static_cast<int>(tile % p.tilesX) * 8 + (lane & 7);
+    const int y = static_cast<int>(tile / p.tilesX) * 8 + (lane >> 3);
+    if (x < p.width && y < p.height) {
+      const RayF r = MakeRay(p, x, y);
+      Counts n{0, 0, 0, 0, 0, 0};
+      const Hit h = Trace<kFast, kCount>(sv, r, p.sc.rootDesc, slab + tid, blockDim.x, cap, table, n);
+      const size_t pix = static_cast<size_t>(y) * p.width + x;
+      ShadeAndStore(p, sv, r, h, pix);
+      if (kCount && p.counters) {
+        int32_t* c = p.counters + 6 * pix;
+        c[0] = n.go; c[1] = n.comp; c[2] = n.box; c[3] = n.sph; c[4] = n.cyl; c[5] = n.cub;
+      }
+    }
+  }
+  // The last wave out returns the queue to zero for the next launch.
+  if (lane == 0) {
+    const uint32_t finished = atomicAdd(p.queue + 1, 1u);
+    if (finished == p.totalWaves - 1) {
+      atomicExch(p.queue, 0u);
+      atomicExch(p.queue + 1, 0u);
+    }
   }
 }
 
-constexpr int kLdsBytesMax = 64 * 1024;
+constexpr int kMaxDepth = 64;
+constexpr size_t kStaticLds = 256;  // the action table and alignment slack
+
+template <bool kFast, bool kCount, bool kLds>
+hipError_t Launch(const RenderParams& p, const LaunchShape& s, hipStream_t stream) {
+  auto fn = RenderKernel<kFast, kCount, kLds>;
+  if (s.lds > 64 * 1024) {
+    const hipError_t e =
+        hipFuncSetAttribute(reinterpret_cast<const void*>(fn), hipFuncAttributeMaxDynamicSharedMemorySize,
+                            static_cast<int>(s.lds));
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(fn, dim3(s.grid), dim3(s.block), s.lds, stream, p);
+  return hipGetLastError();
+}
+
+struct DeviceLimits {
+  int cus = 0;
+  size_t ldsPerBlock = 0;
+};
+
+DeviceLimits Limits(int dev) {
+  DeviceLimits l;
+  int v = 0;
+  if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess) l.cus = v;
+  if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess)
+    l.ldsPerBlock = static_cast<size_t>(v);
+  return l;
+}
 
 }  // namespace
 
-int32_t MaxSupportedDepth() { return kLdsBytesMax / (256 * 8); }
+int32_t MaxSupportedDepth() { return kMaxDepth; }
 
-hipError_t LaunchRender(const RenderParams& p, const LaunchConfig& cfg, hipStream_t stream) {
-  const int block = 256;
-  const dim3 grid((p.width + 15) / 16, (p.height + 15) / 16);
-  const int depth = p.sc.opDepth > 0 ? p.sc.opDepth : 1;
-  const size_t lds = static_cast<size_t>(depth) * block * sizeof(uint2);
+const DeviceLimits* CurrentLimits() {
+  static thread_local int cachedDev = -1;
+  static thread_local DeviceLimits lim;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+  if (dev != cachedDev) {
+    lim = Limits(dev);
+    cachedDev = dev;
+  }
+  return &lim;
+}
+
+hipError_t DeviceShapeLimits(int* computeUnits, size_t* ldsPerBlock) {
+  const DeviceLimits* l = CurrentLimits();
+  if (!l) return hipErrorNoDevice;
+  *computeUnits = l->cus;
+  *ldsPerBlock = l->ldsPerBlock;
+  return hipSuccess;
+}
+
+hipError_t ChooseShape(const DevScene& sc, const LaunchConfig& cfg, LaunchShape* shape) {
+  const DeviceLimits* lp = CurrentLimits();
+  if (!lp) return hipErrorNoDevice;
+  const DeviceLimits& lim = *lp;
+  const int cus = lim.cus > 0 ? lim.cus : 256;
+  const size_t ldsCap = lim.ldsPerBlock > 0 ? lim.ldsPerBlock : 64 * 1024;
+  const size_t depth = static_cast<size_t>(sc.opDepth > 0 ? sc.opDepth : 1);
+  const size_t sceneBytes = (static_cast<size_t>(sc.nOps) * 4 + static_cast<size_t>(sc.nPrims) * 2) * 16;
+  LaunchShape s;
+  bool lds = false;
+  int block = cfg.block > 0 ? cfg.block : 0;
+  if (cfg.sceneInLds != 0) {
+    // the largest workgroup (most waves sharing one scene copy) that fits
+    for (int b : {1024, 512, 256}) {
+      if (block && b != block) continue;
+      if (depth * b * 8 + sceneBytes + kStaticLds <= ldsCap) {
+        lds = true;
+        block = b;
+        break;
+      }
+    }
+  }
+  if (!lds) {
+    if (cfg.sceneInLds == 1) return hipErrorInvalidValue;
+    if (!block) block = 256;
+    while (block > 64 && depth * block * 8 + kStaticLds > ldsCap) block /= 2;
+    if (depth * block * 8 + kStaticLds > ldsCap) return hipErrorInvalidValue;
+  }
+  s.block = block;
+  s.sceneInLds = lds;
+  s.lds = depth * block * 8 + (lds ? sceneBytes : 0);
+  // enough resident waves to fill every CU; extra blocks find the queue empty
+  const int blocksPerCu = lds ? 1 : std::max(1, 2048 / block);
+  s.grid = cus * blocksPerCu;
+  *shape = s;
+  return hipSuccess;
+}
+
+hipError_t LaunchRender(const RenderParams& p, const LaunchConfig& cfg, const LaunchShape& s, hipStream_t stream) {
   const bool fast = cfg.mode == Mode::kFast;
-  if (fast && !cfg.count) hipLaunchKernelGGL((RenderKernel<true, false>), grid, dim3(block), lds, stream, p);
-  else if (fast) hipLaunchKernelGGL((RenderKernel<true, true>), grid, dim3(block), lds, stream, p);
-  else if (!cfg.count) hipLaunchKernelGGL((RenderKernel<false, false>), grid, dim3(block), lds, stream, p);
-  else hipLaunchKernelGGL((RenderKernel<false, true>), grid, dim3(block), lds, stream, p);
-  return hipGetLastError();
+  if (s.sceneInLds) {
+    if (fast) return cfg.count ? Launch<true, true, true>(p, s, stream) : Launch<true, false, true>(p, s, stream);
+    return cfg.count ? Launch<false, true, true>(p, s, stream) : Launch<false, false, true>(p, s, stream);
+  }
+  if (fast) return cfg.count ? Launch<true, true, false>(p, s, stream) : Launch<true, false, false>(p, s, stream);
+  return cfg.count ? Launch<false, true, false>(p, s, stream) : Launch<false, false, false>(p, s, stream);
 }
 
 }  // namespace csgray

@@ -236,17 +236,29 @@ FlatScene Flatten(const CSGTree& tree) {
     std::memcpy(&v, &u, 4);
     return v;
   };
-  f.nodeLo.resize(f.nNodes);
-  f.nodeHi.resize(f.nNodes);
+  // operator index of every operator node, in preorder
+  std::vector<int32_t> opIndex(tree.nodes.size(), -1);
+  for (int32_t i = 0; i < f.nNodes; i++)
+    if (tree.nodes[i].primitiveIdx == -1) opIndex[i] = f.nOps++;
+  auto desc = [&](int32_t node) {
+    const CSGNode& n = tree.nodes[node];
+    const uint32_t index = n.primitiveIdx != -1 ? static_cast<uint32_t>(n.primitiveIdx)
+                                                : static_cast<uint32_t>(opIndex[node]);
+    return static_cast<uint32_t>(n.type) | (index << 8);
+  };
+  f.rootDesc = f.nNodes > 0 ? desc(0) : 0;
+  f.ops.resize(4 * static_cast<size_t>(f.nOps));
   for (int32_t i = 0; i < f.nNodes; i++) {
     const CSGNode& n = tree.nodes[i];
-    const uint32_t prim = n.primitiveIdx < 0 ? 0xFFFFFFu : static_cast<uint32_t>(n.primitiveIdx);
-    const uint32_t meta = static_cast<uint32_t>(n.type) | (prim << 8);
-    const uint32_t right = n.right < 0 ? 0xFFFFu : static_cast<uint32_t>(n.right);
-    const uint32_t parent = n.parent < 0 ? 0xFFFFu : static_cast<uint32_t>(n.parent);
-    const uint32_t link = right | (parent << 16);
-    f.nodeLo[i] = F4{n.box.lo[0], n.box.lo[1], n.box.lo[2], bits(meta)};
-    f.nodeHi[i] = F4{n.box.hi[0], n.box.hi[1], n.box.hi[2], bits(link)};
+    if (n.primitiveIdx != -1) continue;
+    const Box& L = tree.nodes[n.left].box;
+    const Box& R = tree.nodes[n.right].box;
+    const uint32_t parent = n.parent < 0 ? 0xFFFFFFFFu : static_cast<uint32_t>(opIndex[n.parent]);
+    F4* q = &f.ops[4 * static_cast<size_t>(opIndex[i])];
+    q[0] = F4{L.lo[0], L.lo[1], L.lo[2], R.lo[0]};
+    q[1] = F4{L.hi[0], L.hi[1], L.hi[2], R.lo[1]};
+    q[2] = F4{R.lo[2], R.hi[0], R.hi[1], R.hi[2]};
+    q[3] = F4{bits(desc(n.left)), bits(desc(n.right)), bits(static_cast<uint32_t>(n.type)), bits(parent)};
   }
   f.primA.resize(f.nPrims);
   f.primB.resize(f.nPrims);

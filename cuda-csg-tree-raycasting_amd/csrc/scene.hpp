@@ -64,19 +64,28 @@ struct CSGTree {
   void ConstructBVH();
 };
 
-// Device layout, 16-byte records (see the kernels for the readers):
-//   nodeLo[i] = (box.lo.xyz, bits(meta))   meta = type | (primitive << 8)
-//   nodeHi[i] = (box.hi.xyz, bits(link))   link = right | (parent << 16), parent 0xFFFF at the root
-//   primA[p]  = (centre.xyz, radius or cube edge)
-//   primB[p]  = (axis.xyz, height)
-//   primC[p]  = (r, g, b, 0)
+// Device layout.  Traversal only ever stands on operators: a step down reads
+// the operator's record, which holds both children's boxes and descriptors;
+// a step up reads the parent's descriptor word.  Operators are numbered in
+// preorder (0 = the root when the root is an operator).
+//
+//   ops[4k+0] = (L.lo.x, L.lo.y, L.lo.z, R.lo.x)
+//   ops[4k+1] = (L.hi.x, L.hi.y, L.hi.z, R.lo.y)
+//   ops[4k+2] = (R.lo.z, R.hi.x, R.hi.y, R.hi.z)
+//   ops[4k+3] = bits(L.desc, R.desc, type, parent operator or 0xFFFFFFFF)
+//   child descriptor = node type | (index << 8), index = primitive index for a
+//   primitive child, operator index for an operator child
+//   primA[p] = (centre.xyz, radius or cube edge)
+//   primB[p] = (axis.xyz, height)
+//   primC[p] = (r, g, b, 0)
 struct alignas(16) F4 {
   float x, y, z, w;
 };
 
 struct FlatScene {
-  std::vector<F4> nodeLo, nodeHi, primA, primB, primC;
-  int32_t nNodes = 0, nPrims = 0, opDepth = 0;
+  std::vector<F4> ops, primA, primB, primC;
+  uint32_t rootDesc = 0;
+  int32_t nNodes = 0, nOps = 0, nPrims = 0, opDepth = 0;
 };
 
 constexpr int32_t kMaxNodes = 65535;       // 16-bit node links

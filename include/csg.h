@@ -103,8 +103,8 @@ typedef enum csg_mode { CSG_MODE_EXACT = 0, CSG_MODE_FAST = 1 } csg_mode;
 
 typedef struct csg_render_opts {
   int32_t mode;         /* csg_mode */
-  int32_t block;        /* threads per workgroup: 0 = auto, else 64..1024, multiple of 64 */
-  int32_t rays_per_lane;/* persistent refill depth: 0 = auto, 1 = one ray per lane */
+  int32_t block;        /* threads per workgroup: 0 = auto, else 256, 512 or 1024 */
+  int32_t scene_placement; /* 0 = auto, 1 = read the scene from global memory, 2 = stage it in LDS */
   int32_t counters;     /* 1: also write per-ray event counters (testing) */
   int32_t* counter_out; /* device, 6 int32 per pixel: GoTo, Compute, box, sphere, cylinder, cube */
 } csg_render_opts;
@@ -142,6 +142,18 @@ csg_status csg_render_ex(const csg_scene* scene, const csg_camera* cam, const cs
                          const csg_phong* phong, int32_t width, int32_t height, const csg_outputs* out,
                          const csg_render_opts* opts, void* stream);
 csg_status csg_sync(void* stream);
+
+/* Launch geometry csg_render_ex would use for this scene and options on the
+ * current device (no reference counterpart; for tests and tuning). */
+typedef struct csg_launch_info {
+  int32_t block;          /* threads per workgroup */
+  int32_t grid;           /* workgroups (persistent: waves pull 8x8 tiles from a queue) */
+  int64_t lds_bytes;      /* dynamic LDS per workgroup */
+  int32_t scene_in_lds;   /* 1 if the scene is staged in LDS */
+  int32_t compute_units;
+  int64_t lds_limit;      /* LDS per workgroup the device allows */
+} csg_launch_info;
+csg_status csg_render_plan(const csg_scene* scene, const csg_render_opts* opts, csg_launch_info* info);
 
 const char* csg_last_error(void);
 const char* csg_version(void);
