@@ -54,7 +54,7 @@ class SceneInfo(ctypes.Structure):
 
 class RenderOpts(ctypes.Structure):
     _fields_ = [("mode", ctypes.c_int32), ("block", ctypes.c_int32), ("scene_placement", ctypes.c_int32),
-                ("counters", ctypes.c_int32), ("counter_out", ctypes.c_void_p)]
+                ("counters", ctypes.c_int32), ("counter_out", ctypes.c_void_p), ("tile_order", ctypes.c_void_p)]
 
 
 class LaunchInfo(ctypes.Structure):
@@ -221,12 +221,12 @@ class Scene:
 
     def plan(self, mode=MODE_FAST, block=0, placement=0) -> LaunchInfo:
         info = LaunchInfo()
-        opts = RenderOpts(mode, block, placement, 0, None)
+        opts = RenderOpts(mode, block, placement, 0, None, None)
         _check(lib().csg_render_plan(self.handle, ctypes.byref(opts), ctypes.byref(info)))
         return info
 
     def render(self, cam, light_, phong_, width, height, mode=MODE_FAST, outputs=None, counters=False,
-               stream=None, device=None, block=0, placement=0):
+               stream=None, device=None, block=0, placement=0, tile_order=None):
         """Render into torch CUDA tensors; returns dict rgba/depth/prim_id/flags(/counters).
 
         `outputs` may pass preallocated tensors (same keys) to avoid allocation."""
@@ -248,7 +248,8 @@ class Scene:
         o = Outputs(*(outputs[k].data_ptr() if outputs.get(k) is not None else None
                       for k in ("rgba", "depth", "prim_id", "flags")))
         opts = RenderOpts(mode, block, placement, 1 if counters else 0,
-                          outputs["counters"].data_ptr() if counters else None)
+                          outputs["counters"].data_ptr() if counters else None,
+                          tile_order.data_ptr() if tile_order is not None else None)
         s = stream if stream is not None else torch.cuda.current_stream(dev).cuda_stream
         _check(lib().csg_render_ex(self.handle, ctypes.byref(cam), ctypes.byref(light_), ctypes.byref(phong_),
                                    int(width), int(height), ctypes.byref(o), ctypes.byref(opts), s))

@@ -367,6 +367,7 @@ csg_status csg_render_ex(const csg_scene* s, const csg_camera* cam, const csg_li
   p.primId = out->prim_id;
   p.flags = out->flags;
   p.counters = cfg.count ? opts->counter_out : nullptr;
+  p.tileOrder = opts ? opts->tile_order : nullptr;
   p.tilesX = (width + 7) / 8;
   const int64_t tiles = static_cast<int64_t>(p.tilesX) * ((height + 7) / 8);
   p.numTiles = static_cast<int32_t>(tiles);

@@ -33,6 +33,7 @@ struct RenderParams {
   int32_t tilesX, numTiles;  // 8x8-pixel tiles, row-major
   uint32_t totalWaves;       // waves in the grid (queue reset bookkeeping)
   uint32_t* queue;           // [0] next tile, [1] waves finished; zero between launches
+  const uint32_t* tileOrder; // optional: the order in which tiles are taken (numTiles entries)
   float4* rgba;
   float* depth;
   int32_t* primId;

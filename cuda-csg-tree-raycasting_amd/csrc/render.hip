@@ -307,12 +307,17 @@ __device__ __forceinline__ Hit Intersect(const SceneView& sv, const RayF& r, uin
 // GetParent (:708-716).  `stk` points at this lane's slot 0; slot s is at
 // stk[s * stride].
 //
-// The step is written for SIMT execution: decisions are selects rather than
-// nested branches, stack writes and reads are unconditional within their
-// step (indices clamped to the slab, an out-of-range access marks the ray
-// aborted instead of branching), and the primitive tests of a step run
-// through one call site.  Lanes of a wave sit at different points of their
-// own state machine, so every branch a lane takes costs the whole wave.
+// The step is written for SIMT execution and for the latency of one wave
+// running alone (the longest ray of a frame sets its time):
+//   * every loop iteration is exactly one micro-step, a GoTo or a Compute
+//     (the reference may run SaveLft, GoTo and Compute in one iteration; the
+//     sequence of GoTo and Compute calls is the same);
+//   * SaveLft is folded into the Compute that pops it;
+//   * a Compute issues its three LDS reads (action table, stack top, parent
+//     descriptor) at once, before it knows whether it pops;
+//   * decisions are selects and non-short-circuit boolean logic; stack
+//     indices are clamped to the slab and an out-of-range access marks the
+//     ray aborted instead of branching.
 template <bool kFast, bool kCount>
 __device__ Hit Trace(const SceneView& sv, const RayF& r, uint32_t rootDesc, uint2* stk, int stride, int cap,
                      const uint16_t* table, Counts& n) {
@@ -331,26 +336,30 @@ __device__ Hit Trace(const SceneView& sv, const RayF& r, uint32_t rootDesc, uint
   // The operator the machine stands on.  Initially the reference's virtual
   // parent of the root, whose left child is the root.
   uint32_t opType = 0, opParent = kNone, lDesc = rootDesc, rDesc = 0;
+  // At the top of the loop action is GotoLft, GotoRgh, Compute, LoadLft or
+  // LoadRgh.
   int action = kGotoLft;
   int sp = 0;
   bool done = false, aborted = false;
   int iter = 0;
 
   while (!done) {
-    if (action == kSaveLft) {
-      // tmin saved by the GoTo that descended left comes back, and the left
-      // result takes its slot (the slot was popped by the Compute that
-      // returned here).
-      tmin = held.t;
-      stk[sp * stride] = make_uint2(AsU(L.t), (L.bits & ~kSlotMask) | kSlotLoadLft);
-      sp++;
-      action = kGotoRgh;
-    }
-    if (action == kGotoLft || action == kGotoRgh) {
+    if (action <= kGotoRgh) {
       if (kCount) n.go++;
       const bool left = action == kGotoLft;
       const uint32_t d = left ? lDesc : rDesc;
-      if (DescType(d) < kSphereT) {
+      if (DescType(d) >= kSphereT) {
+        // A primitive child re-entered by a loop: intersect beyond tmin,
+        // then pop the Load the loop left for this operator.
+        const Hit h = Intersect<kFast, kCount>(sv, r, d, tmin, left ? L : R, n);
+        L = left ? h : L;
+        R = left ? R : h;
+        aborted |= sp <= 0;
+        sp = max(sp - 1, 0);
+        const uint2 e = stk[sp * stride];
+        held = Hit{__uint_as_float(e.x), e.y & ~kSlotMask};
+        action = (e.y & kSlotMask) == kSlotLoadLft ? kLoadLft : kLoadRgh;
+      } else {
         const uint32_t k = DescIndex(d);
         const float4 q0 = sv.ops[4 * k + 0];
         const float4 q1 = sv.ops[4 * k + 1];
@@ -365,9 +374,9 @@ __device__ Hit Trace(const SceneView& sv, const RayF& r, uint32_t rootDesc, uint
         if (kCount) n.box += 2;
         L.bits = gL ? L.bits : WithFlags(L.bits, kMissBit);
         R.bits = gR ? R.bits : WithFlags(R.bits, kMissBit);
-        const bool pL = gL && DescType(lDesc) >= kSphereT;
-        const bool pR = gR && DescType(rDesc) >= kSphereT;
-        if (pL || pR) {
+        const bool pL = gL & (DescType(lDesc) >= kSphereT);
+        const bool pR = gR & (DescType(rDesc) >= kSphereT);
+        if (pL | pR) {
 #pragma nounroll
           for (int j = 0; j < 2; j++) {
             if (j == 0 ? pL : pR) {
@@ -377,44 +386,40 @@ __device__ Hit Trace(const SceneView& sv, const RayF& r, uint32_t rootDesc, uint
             }
           }
         }
-        const bool dL = gL && !pL, dR = gR && !pR;
+        const bool dL = gL & !pL, dR = gR & !pR;
         // both -> park tmin (LoadLft+SaveLft); right only -> park L; left only -> park R
-        const bool both = dL && dR;
+        const bool both = dL & dR;
         const float pt = both ? tmin : (dR ? L.t : R.t);
         const uint32_t pb = both ? kSlotSaveLft
                                  : (dR ? ((L.bits & ~kSlotMask) | kSlotLoadLft) : ((R.bits & ~kSlotMask) | kSlotLoadRgh));
-        if (dL || dR) {
+        if (dL | dR) {
           aborted |= sp >= cap;
           stk[min(sp, cap - 1) * stride] = make_uint2(AsU(pt), pb);
           sp++;
         }
         action = dL ? kGotoLft : (dR ? kGotoRgh : kCompute);
-      } else {
-        // A primitive child re-entered by a loop: intersect beyond tmin and
-        // hand the record to this operator's Compute (its loop left a Load
-        // on the stack).
-        const Hit h = Intersect<kFast, kCount>(sv, r, d, tmin, left ? L : R, n);
-        if (left) L = h;
-        else R = h;
-        aborted |= sp <= 0;
-        sp = max(sp - 1, 0);
-        const uint2 e = stk[sp * stride];
-        held = Hit{__uint_as_float(e.x), e.y & ~kSlotMask};
-        const uint32_t code = e.y & kSlotMask;
-        action = code == kSlotLoadLft ? kLoadLft : (code == kSlotLoadRgh ? kLoadRgh : kSaveLft);
       }
-    }
-    if (action == kCompute || action == kLoadLft || action == kLoadRgh) {
+    } else {
       if (kCount) n.comp++;
-      if (action == kLoadLft) L = held;
-      if (action == kLoadRgh) R = held;
+      L = action == kLoadLft ? held : L;
+      R = action == kLoadRgh ? held : R;
+      // Speculative reads, all issued before the decision: the action, the
+      // stack top (popped if the operator returns) and the parent's
+      // descriptor word (the root reads operator 0's and ignores it).
       const uint32_t acts = table[opType * 9u + State(L.bits) * 3u + State(R.bits)];
+      const int top = min(max(sp - 1, 0), cap - 1);
+      const uint2 e = stk[top * stride];
+      const float4 pq = sv.ops[4 * (opParent == kNone ? 0u : opParent) + 3];
       const bool lc = L.t < R.t, rc = L.t > R.t;
-      const bool retL = (acts & kRetL) || ((acts & kRetLIfCloser) && lc);
-      const bool retR = !retL && ((acts & kRetR) || ((acts & kRetRIfCloser) && rc));
-      const bool loopL = !retL && !retR && ((acts & kLoopL) || ((acts & kLoopLIfCloser) && lc));
-      const bool loopR = !retL && !retR && !loopL && ((acts & kLoopR) || ((acts & kLoopRIfCloser) && rc));
-      if (loopL || loopR) {
+      const bool aRetL = (acts & kRetL) != 0, aRetLc = (acts & kRetLIfCloser) != 0;
+      const bool aRetR = (acts & kRetR) != 0, aRetRc = (acts & kRetRIfCloser) != 0;
+      const bool aLoopL = (acts & kLoopL) != 0, aLoopLc = (acts & kLoopLIfCloser) != 0;
+      const bool aLoopR = (acts & kLoopR) != 0, aLoopRc = (acts & kLoopRIfCloser) != 0;
+      const bool retL = aRetL | (aRetLc & lc);
+      const bool retR = !retL & (aRetR | (aRetRc & rc));
+      const bool loopL = !retL & !retR & (aLoopL | (aLoopLc & lc));
+      const bool loopR = !retL & !retR & !loopL & (aLoopR | (aLoopRc & rc));
+      if (loopL | loopR) {
         tmin = loopL ? L.t : R.t;
         const Hit park = loopL ? R : L;
         aborted |= sp >= cap;
@@ -427,25 +432,29 @@ __device__ Hit Trace(const SceneView& sv, const RayF& r, uint32_t rootDesc, uint
         const Hit res = retL ? L : (retR ? Hit{R.t, rb} : MissRecord());
         L = res;
         R = res;
-        if (opParent == kNone) {
-          done = true;
+        done = opParent == kNone;
+        aborted |= !done & (sp <= 0);
+        sp = top;
+        held = Hit{__uint_as_float(e.x), e.y & ~kSlotMask};
+        const uint32_t code = e.y & kSlotMask;
+        lDesc = AsU(pq.x);
+        rDesc = AsU(pq.y);
+        opType = AsU(pq.z);
+        opParent = AsU(pq.w);
+        if (code == kSlotSaveLft) {
+          // SaveLft: tmin parked by the GoTo that went left comes back and
+          // the left result takes its slot; then go right.
+          tmin = held.t;
+          stk[sp * stride] = make_uint2(AsU(L.t), (L.bits & ~kSlotMask) | kSlotLoadLft);
+          sp++;
+          action = kGotoRgh;
         } else {
-          const float4 q3 = sv.ops[4 * opParent + 3];
-          aborted |= sp <= 0;
-          sp = max(sp - 1, 0);
-          const uint2 e = stk[sp * stride];
-          held = Hit{__uint_as_float(e.x), e.y & ~kSlotMask};
-          const uint32_t code = e.y & kSlotMask;
-          action = code == kSlotLoadLft ? kLoadLft : (code == kSlotLoadRgh ? kLoadRgh : kSaveLft);
-          lDesc = AsU(q3.x);
-          rDesc = AsU(q3.y);
-          opType = AsU(q3.z);
-          opParent = AsU(q3.w);
+          action = code == kSlotLoadLft ? kLoadLft : kLoadRgh;
         }
       }
     }
     iter++;
-    done = done || aborted || iter >= kIterCap;
+    done = done | aborted | (iter >= kIterCap);
   }
   if (aborted || iter >= kIterCap) L = Hit{-1.0f, kMissBit | kAborted | 0xFFFF0000u};
   return L;
@@ -590,6 +599,7 @@ __global__ void __launch_bounds__(1024) RenderKernel(RenderParams p) {
     if (lane == 0) tile = atomicAdd(p.queue, 1u);
     tile = __builtin_amdgcn_readfirstlane(tile);
     if (tile >= static_cast<uint32_t>(p.numTiles)) break;
+    if (p.tileOrder) tile = p.tileOrder[tile];
     const int x = static_cast<int>(tile % p.tilesX) * 8 + (lane & 7);
     const int y = static_cast<int>(tile / p.tilesX) * 8 + (lane >> 3);
     if (x < p.width && y < p.height) {

@@ -107,6 +107,8 @@ typedef struct csg_render_opts {
   int32_t scene_placement; /* 0 = auto, 1 = read the scene from global memory, 2 = stage it in LDS */
   int32_t counters;     /* 1: also write per-ray event counters (testing) */
   int32_t* counter_out; /* device, 6 int32 per pixel: GoTo, Compute, box, sphere, cylinder, cube */
+  const uint32_t* tile_order; /* device, optional: order in which the ceil(w/8) x ceil(h/8) tiles of
+                                 8x8 pixels (row-major ids) are taken; a permutation, or NULL */
 } csg_render_opts;
 
 /* --- scene (replaces CSGTree::Parse + the upload in Raycaster::ChangeSize) --- */

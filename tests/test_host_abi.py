@@ -1,0 +1,114 @@
+"""The C ABI library on the host: it loads without a GPU, exports every
+function include/csg.h declares, and its host-side pieces (camera model,
+light and shading defaults, scene info, argument checks) match the
+reference's definitions and the oracle."""
+import math
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import csgray as C
+import oracle as O
+from common import DEFAULT_CAM6
+from conftest import ROOT, scene_text
+
+HEADER = os.path.join(ROOT, "include", "csg.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(csg_\w+)\s*\(", text)))
+
+
+def test_header_declares_what_the_binding_binds():
+    assert declared_functions() == sorted(C.symbols())
+
+
+def test_library_exports_every_declared_symbol():
+    C.lib()
+    out = subprocess.run(["nm", "-D", "--defined-only", C.LIB_PATH], capture_output=True, text=True,
+                         check=True).stdout
+    exported = set(re.findall(r"\sT\s(csg_\w+)", out))
+    missing = [f for f in declared_functions() if f not in exported]
+    assert not missing
+
+
+def test_header_compiles_as_c():
+    src = "#include \"csg.h\"\nint main(void){csg_camera c; csg_camera_default(&c); return (int)c.pos[2] - 5;}\n"
+    p = subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-fsyntax-only", "-I", os.path.dirname(HEADER),
+                        "-x", "c", "-"], input=src, text=True, capture_output=True)
+    assert p.returncode == 0, p.stderr
+
+
+def test_version_and_last_error():
+    assert b"gfx950" in C.lib().csg_version()
+    with pytest.raises(C.CsgError):
+        C.Scene("Bogus 1 2 3", upload=False)
+    assert b"Unrecognized keyword" in C.lib().csg_last_error()
+
+
+def test_default_camera_matches_reference():
+    cam = C.camera_default()
+    assert tuple(cam.pos) == (0.0, 0.0, 5.0) and cam.pitch == 0.0 and cam.yaw == 0.0
+    assert cam.vfov == np.float32(np.float32(90.0) * np.float32(3.14159) / np.float32(180.0))
+    b = C.camera_basis(cam)
+    assert np.allclose(b[0:3], [0, 0, -1]) and np.allclose(b[3:6], [1, 0, 0]) and np.allclose(b[6:9], [0, -1, 0])
+
+
+@pytest.mark.parametrize("cam6", [DEFAULT_CAM6, (3.0, 2.5, 4.0, -0.4636476, 0.6435011, 1.0471975),
+                                  (-2.0, 7.0, 1.0, 1.2, -2.0, 0.5), (0.0, 0.0, 9.0, 2.0, 0.3, 1.7)])
+def test_camera_basis_bitwise_equal_to_oracle(cam6):
+    mine = np.array(C.camera_basis(C.camera(cam6[:3], cam6[3], cam6[4], cam6[5])), np.float32)
+    ref = O.camera_basis(cam6, bits=32).astype(np.float32)
+    assert np.array_equal(mine.view(np.uint32), ref.view(np.uint32))
+
+
+def test_pitch_is_clamped_like_set_rotation():
+    lim = np.float32(np.float32(89.0) * np.float32(3.14159) / np.float32(180.0))
+    up = C.camera_basis(C.camera((0, 0, 0), 2.0, 0.0, 1.0))
+    assert abs(up[1] - math.sin(lim)) < 1e-6
+
+
+def test_look_at_recovers_the_view_configuration():
+    cam = C.camera_look_at((3, 2.5, 4), (0, 0, 0), 60)
+    assert abs(cam.pitch - (-0.4636476)) < 1e-6 and abs(cam.yaw - 0.6435011) < 1e-6
+    assert cam.vfov == np.float32(np.float32(60.0) * np.float32(3.14159) / np.float32(180.0))
+    f = np.array(C.camera_basis(cam)[0:3])
+    assert np.allclose(f, -np.array([3, 2.5, 4]) / math.sqrt(31.25), atol=1e-6)
+    with pytest.raises(C.CsgError):
+        C.camera_look_at((1, 1, 1), (1, 1, 1), 60)
+
+
+def test_light_and_phong_defaults():
+    l = C.light_default()
+    p, a = np.float32(-60.0 * 3.14159 / 180.0), np.float32(-45.0 * 3.14159 / 180.0)
+    assert np.allclose(list(l.dir), [math.sin(p) * math.cos(a), math.cos(p), math.sin(p) * math.sin(a)], atol=1e-6)
+    ph = C.phong_default()
+    assert (ph.ka, ph.kd, ph.ks, ph.shininess) == (np.float32(0.2), np.float32(0.8), np.float32(0.7), 30.0)
+    assert np.allclose(list(ph.bg), [0.08, 0.08, 0.11, 1.0])
+
+
+def test_scene_info_counts():
+    s = C.Scene(scene_text("testCubeCutEdges.txt"), upload=False)
+    i = s.info()
+    assert (i.nodes, i.primitives, i.op_depth, i.spheres, i.cylinders, i.cubes) == (25, 13, 5, 0, 12, 1)
+    assert i.device_bytes == 0
+
+
+def test_render_refuses_host_only_scene_without_touching_the_gpu():
+    s = C.Scene(scene_text("test1.txt"), upload=False)
+    o = C.Outputs(None, None, None, None)
+    st = C.lib().csg_render(s.handle, C.camera_default(), C.light_default(), C.phong_default(), 16, 16, o, None)
+    assert st == C.E_ARG
+
+
+def test_null_arguments_are_errors_not_crashes():
+    L = C.lib()
+    assert L.csg_scene_info_get(None, None) == C.E_ARG
+    assert L.csg_scene_export(None, None, None, None) == C.E_ARG
+    assert L.csg_scene_free(None) == C.OK
+    assert L.csg_render(None, None, None, None, 4, 4, None, None) == C.E_ARG
