@@ -54,7 +54,13 @@ class SceneInfo(ctypes.Structure):
 
 class RenderOpts(ctypes.Structure):
     _fields_ = [("mode", ctypes.c_int32), ("block", ctypes.c_int32), ("scene_placement", ctypes.c_int32),
-                ("counters", ctypes.c_int32), ("counter_out", ctypes.c_void_p), ("tile_order", ctypes.c_void_p)]
+                ("counters", ctypes.c_int32), ("counter_out", ctypes.c_void_p), ("tile_order", ctypes.c_void_p),
+                ("samples", ctypes.c_int32), ("seed", ctypes.c_uint32), ("tile_walk", ctypes.c_int32)]
+
+
+class Job(ctypes.Structure):
+    _fields_ = [("scene", ctypes.c_void_p), ("camera", Camera), ("light", Light), ("phong", Phong),
+                ("width", ctypes.c_int32), ("height", ctypes.c_int32), ("out", Outputs)]
 
 
 class LaunchInfo(ctypes.Structure):
@@ -94,6 +100,9 @@ _SIGNATURES = [
                                      ctypes.POINTER(RenderOpts), _P]),
     ("csg_sync", ctypes.c_int, [_P]),
     ("csg_render_plan", ctypes.c_int, [_P, ctypes.POINTER(RenderOpts), ctypes.POINTER(LaunchInfo)]),
+    ("csg_jitter", ctypes.c_float, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32]),
+    ("csg_render_batch", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(RenderOpts), _P]),
+    ("csg_batch_order", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p]),
     ("csg_last_error", ctypes.c_char_p, []),
     ("csg_version", ctypes.c_char_p, []),
 ]
@@ -221,27 +230,21 @@ class Scene:
 
     def plan(self, mode=MODE_FAST, block=0, placement=0) -> LaunchInfo:
         info = LaunchInfo()
-        opts = RenderOpts(mode, block, placement, 0, None, None)
+        opts = RenderOpts(mode, block, placement, 0, None, None, 0, 0, 0)
         _check(lib().csg_render_plan(self.handle, ctypes.byref(opts), ctypes.byref(info)))
         return info
 
     def render(self, cam, light_, phong_, width, height, mode=MODE_FAST, outputs=None, counters=False,
-               stream=None, device=None, block=0, placement=0, tile_order=None):
+               stream=None, device=None, block=0, placement=0, tile_order=None, samples=1, seed=0, walk=0):
         """Render into torch CUDA tensors; returns dict rgba/depth/prim_id/flags(/counters).
 
         `outputs` may pass preallocated tensors (same keys) to avoid allocation."""
         import torch
         dev = device or torch.device("cuda", torch.cuda.current_device())
-        n = width * height
+        spp = max(1, int(samples))
+        n = width * height * spp
         if outputs is None:
-            outputs = {
-                "rgba": torch.empty((height, width, 4), dtype=torch.float32, device=dev),
-                "depth": torch.empty((height, width), dtype=torch.float32, device=dev),
-                "prim_id": torch.empty((height, width), dtype=torch.int32, device=dev),
-                "flags": torch.empty((height, width), dtype=torch.uint8, device=dev),
-            }
-            if counters:
-                outputs["counters"] = torch.zeros((height, width, 6), dtype=torch.int32, device=dev)
+            outputs = alloc_outputs(width, height, spp, dev, counters)
         for k, t in outputs.items():
             assert t.is_cuda and t.is_contiguous(), k
         assert outputs["rgba"].numel() >= 4 * n if "rgba" in outputs else True
@@ -249,11 +252,58 @@ class Scene:
                       for k in ("rgba", "depth", "prim_id", "flags")))
         opts = RenderOpts(mode, block, placement, 1 if counters else 0,
                           outputs["counters"].data_ptr() if counters else None,
-                          tile_order.data_ptr() if tile_order is not None else None)
+                          tile_order.data_ptr() if tile_order is not None else None, spp, seed, walk)
         s = stream if stream is not None else torch.cuda.current_stream(dev).cuda_stream
         _check(lib().csg_render_ex(self.handle, ctypes.byref(cam), ctypes.byref(light_), ctypes.byref(phong_),
                                    int(width), int(height), ctypes.byref(o), ctypes.byref(opts), s))
         return outputs
+
+
+def alloc_outputs(width, height, spp=1, device=None, counters=False):
+    """Output tensors for one render: (H, W*spp, ...) row-major, sample s of pixel x at column x*spp+s."""
+    import torch
+    dev = device or torch.device("cuda", torch.cuda.current_device())
+    w = width * spp
+    out = {
+        "rgba": torch.empty((height, w, 4), dtype=torch.float32, device=dev),
+        "depth": torch.empty((height, w), dtype=torch.float32, device=dev),
+        "prim_id": torch.empty((height, w), dtype=torch.int32, device=dev),
+        "flags": torch.empty((height, w), dtype=torch.uint8, device=dev),
+    }
+    if counters:
+        out["counters"] = torch.zeros((height, w, 6), dtype=torch.int32, device=dev)
+    return out
+
+
+def _outputs_struct(outputs):
+    return Outputs(*(outputs[k].data_ptr() if outputs.get(k) is not None else None
+                     for k in ("rgba", "depth", "prim_id", "flags")))
+
+
+def jitter(seed, pixel, sample, axis):
+    return float(lib().csg_jitter(seed, pixel, sample, axis))
+
+
+def make_jobs(specs):
+    """specs: list of (Scene, cam, light, phong, width, height, outputs dict) -> ctypes array of Job."""
+    arr = (Job * len(specs))()
+    for i, (scene, cam, light_, phong_, w, h, outs) in enumerate(specs):
+        arr[i] = Job(scene.handle, cam, light_, phong_, int(w), int(h), _outputs_struct(outs))
+    return arr
+
+
+def batch_order(jobs):
+    order = (ctypes.c_int32 * len(jobs))()
+    _check(lib().csg_batch_order(jobs, len(jobs), order))
+    return list(order)
+
+
+def render_batch(jobs, mode=MODE_FAST, block=0, walk=0, samples=1, seed=0, stream=None, counters_out=None):
+    """One persistent launch over a ctypes Job array (see make_jobs)."""
+    import torch
+    opts = RenderOpts(mode, block, PLACE_GLOBAL, 0, None, None, samples, seed, walk)
+    s = stream if stream is not None else torch.cuda.current_stream().cuda_stream
+    _check(lib().csg_render_batch(jobs, len(jobs), ctypes.byref(opts), s))
 
 
 @dataclass

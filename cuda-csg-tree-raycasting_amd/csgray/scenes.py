@@ -118,6 +118,20 @@ def orbit_eye(rng, radius=4.0, max_elev_deg=80.0):
     return (radius * math.cos(el) * math.sin(az), radius * math.sin(el), radius * math.cos(el) * math.cos(az))
 
 
+def batch_mix(n=256, seed=256, resolutions=(128, 256, 512, 1024)):
+    """The batched mix: n random trees (see random_tree), each with a random
+    orbit camera at radius 4 looking at the origin and a square resolution
+    drawn uniformly from `resolutions`.  Returns [(text, eye, size)]."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        text = random_tree(rng)
+        eye = orbit_eye(rng, 4.0, 85.0)
+        size = int(resolutions[int(rng.integers(0, len(resolutions)))])
+        out.append((text, eye, size))
+    return out
+
+
 # ---- named configurations --------------------------------------------------
 
 # Shared camera / light / shading of the Wikipedia and cheese configurations:

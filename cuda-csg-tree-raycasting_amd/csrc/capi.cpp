@@ -302,8 +302,117 @@ csg_status ParseOpts(const csg_render_opts* opts, csgray::LaunchConfig* cfg) {
     return Fail(CSG_E_ARG, "csg_render: block must be 0, 256, 512 or 1024");
   if (opts->scene_placement < 0 || opts->scene_placement > 2)
     return Fail(CSG_E_ARG, "csg_render: scene_placement must be 0, 1 or 2");
+  if (opts->samples < 0 || opts->samples > 1024) return Fail(CSG_E_ARG, "csg_render: samples must be in [0, 1024]");
+  if (opts->tile_walk != 0 && opts->tile_walk != 1) return Fail(CSG_E_ARG, "csg_render: tile_walk must be 0 or 1");
   cfg->block = opts->block;
   cfg->sceneInLds = opts->scene_placement == 0 ? -1 : (opts->scene_placement == 2 ? 1 : 0);
+  cfg->morton = opts->tile_walk == 1;
+  return CSG_OK;
+}
+
+size_t SceneBytes(const csg_scene* s) {
+  return (static_cast<size_t>(s->dev.nOps) * 4 + static_cast<size_t>(s->dev.nPrims) * 2) * 16;
+}
+
+// Validates one job and fills its device descriptor (outputs, camera basis,
+// light, shading, tile counts).  `tileBase` is its first queue index.
+csg_status BuildJob(const csg_scene* s, const csg_camera* cam, const csg_light* light, const csg_phong* phong,
+                    int32_t width, int32_t height, const csg_outputs* out, const csg_render_opts* opts,
+                    int dev, int64_t tileBase, csgray::Job* jb) {
+  if (!s || !cam || !light || !phong || !out) return Fail(CSG_E_ARG, "csg_render: NULL argument");
+  // The reference's ray generation divides by width-1 and height-1
+  // (RaycastingKernels.cu:11-12), so both must be at least 2.
+  if (width < 2 || height < 2) return Fail(CSG_E_ARG, "csg_render: width and height must be >= 2");
+  const int32_t spp = opts && opts->samples > 1 ? opts->samples : 1;
+  const int64_t rays = static_cast<int64_t>(width) * height * spp;
+  if (rays > (int64_t{1} << 31)) return Fail(CSG_E_ARG, "csg_render: more than 2^31 rays");
+  if (!s->dmem) return Fail(CSG_E_ARG, "csg_render: scene is not on a device (call csg_scene_upload)");
+  if (dev != s->device) return Fail(CSG_E_ARG, "csg_render: scene was uploaded to another device");
+  std::memset(jb, 0, sizeof(*jb));
+  jb->sc = s->dev;
+  const csgray::CameraBasis b = csgray::MakeBasis(cam->pos, cam->pitch, cam->yaw, cam->vfov);
+  for (int k = 0; k < 3; k++) {
+    jb->pos[k] = b.pos[k];
+    jb->forward[k] = b.forward[k];
+    jb->right[k] = b.right[k];
+    jb->up[k] = b.up[k];
+  }
+  jb->tanHalf = b.tanHalf;
+  csgray::NormalizeLight(light->dir, jb->light);
+  jb->ka = phong->ka;
+  jb->kd = phong->kd;
+  jb->ks = phong->ks;
+  jb->shininess = phong->shininess;
+  for (int k = 0; k < 4; k++) jb->bg[k] = phong->bg[k];
+  jb->width = width;
+  jb->height = height;
+  jb->tilesX = (width + 7) / 8;
+  const int64_t tiles = static_cast<int64_t>(jb->tilesX) * ((height + 7) / 8) * spp;
+  if (tileBase + tiles >= (int64_t{1} << 31)) return Fail(CSG_E_ARG, "csg_render: too many tiles in one launch");
+  jb->tiles = static_cast<int32_t>(tiles);
+  jb->tileBase = static_cast<int32_t>(tileBase);
+  jb->spp = spp;
+  jb->seed = opts ? opts->seed : 0u;
+  jb->rgba = reinterpret_cast<float4*>(out->rgba);
+  jb->depth = out->depth;
+  jb->primId = out->prim_id;
+  jb->flags = out->flags;
+  jb->counters = (opts && opts->counters) ? opts->counter_out : nullptr;
+  return CSG_OK;
+}
+
+// Per-(device, stream) job-table buffers for batched launches: a device
+// table and a pinned host staging copy, grown on demand.  `copied` marks the
+// last upload; the staging copy is rewritten only after it completes.
+struct JobTable {
+  void* dev = nullptr;
+  void* host = nullptr;
+  size_t cap = 0;
+  hipEvent_t copied = nullptr;
+};
+std::mutex g_jobsMutex;
+std::map<std::pair<int, hipStream_t>, JobTable> g_jobTables;
+
+csg_status UploadJobs(int dev, hipStream_t stream, const csgray::Job* jobs, size_t n, const csgray::Job** out) {
+  std::lock_guard<std::mutex> lock(g_jobsMutex);
+  JobTable& t = g_jobTables[{dev, stream}];
+  const size_t bytes = sizeof(csgray::Job) * n;
+  hipError_t e;
+  if (t.cap < bytes) {
+    if (t.dev) {
+      // earlier launches on this stream may still read the old table
+      e = hipStreamSynchronize(stream);
+      if (e != hipSuccess) return FailHip(e, "hipStreamSynchronize");
+      (void)hipFree(t.dev);
+      (void)hipHostFree(t.host);
+      t.dev = t.host = nullptr;
+      t.cap = 0;
+    }
+    const size_t cap = std::max<size_t>(bytes, 64 * 1024);
+    e = hipMalloc(&t.dev, cap);
+    if (e != hipSuccess) return e == hipErrorOutOfMemory ? Fail(CSG_E_OOM, "job table allocation failed")
+                                                         : FailHip(e, "hipMalloc");
+    e = hipHostMalloc(&t.host, cap, hipHostMallocDefault);
+    if (e != hipSuccess) {
+      (void)hipFree(t.dev);
+      t.dev = nullptr;
+      return Fail(CSG_E_OOM, "pinned job staging allocation failed");
+    }
+    t.cap = cap;
+    if (!t.copied) {
+      e = hipEventCreateWithFlags(&t.copied, hipEventDisableTiming);
+      if (e != hipSuccess) return FailHip(e, "hipEventCreate");
+    }
+  } else if (t.copied) {
+    e = hipEventSynchronize(t.copied);  // the staging copy is free once the last upload is done
+    if (e != hipSuccess) return FailHip(e, "hipEventSynchronize");
+  }
+  std::memcpy(t.host, jobs, bytes);
+  e = hipMemcpyAsync(t.dev, t.host, bytes, hipMemcpyHostToDevice, stream);
+  if (e != hipSuccess) return FailHip(e, "hipMemcpyAsync of the job table");
+  e = hipEventRecord(t.copied, stream);
+  if (e != hipSuccess) return FailHip(e, "hipEventRecord");
+  *out = static_cast<const csgray::Job*>(t.dev);
   return CSG_OK;
 }
 }  // namespace
@@ -318,7 +427,7 @@ csg_status csg_render_plan(const csg_scene* s, const csg_render_opts* opts, csg_
   hipError_t e = csgray::DeviceShapeLimits(&cus, &ldsLimit);
   if (e != hipSuccess) return FailHip(e, "device query");
   csgray::LaunchShape shape;
-  e = csgray::ChooseShape(s->dev, cfg, &shape);
+  e = csgray::ChooseShape(s->dev.opDepth, SceneBytes(s), cfg, &shape);
   if (e != hipSuccess) return FailHip(e, "choosing the launch shape");
   info->block = shape.block;
   info->grid = shape.grid;
@@ -329,50 +438,31 @@ csg_status csg_render_plan(const csg_scene* s, const csg_render_opts* opts, csg_
   return CSG_OK;
 }
 
+float csg_jitter(uint32_t seed, uint32_t pixel, uint32_t sample, uint32_t axis) {
+  return csgray::JitterUnit(csgray::JitterHash(seed, pixel, sample, axis));
+}
+
 csg_status csg_render_ex(const csg_scene* s, const csg_camera* cam, const csg_light* light, const csg_phong* phong,
                          int32_t width, int32_t height, const csg_outputs* out, const csg_render_opts* opts,
                          void* stream) {
-  if (!s || !cam || !light || !phong || !out) return Fail(CSG_E_ARG, "csg_render: NULL argument");
-  // The reference's ray generation divides by width-1 and height-1
-  // (RaycastingKernels.cu:11-12), so both must be at least 2.
-  if (width < 2 || height < 2) return Fail(CSG_E_ARG, "csg_render: width and height must be >= 2");
-  if (static_cast<int64_t>(width) * height > (int64_t{1} << 31))
-    return Fail(CSG_E_ARG, "csg_render: image larger than 2^31 pixels");
   csgray::LaunchConfig cfg;
   const csg_status ost = ParseOpts(opts, &cfg);
   if (ost != CSG_OK) return ost;
+  if (!s || !cam || !light || !phong || !out) return Fail(CSG_E_ARG, "csg_render: NULL argument");
   if (!s->dmem) return Fail(CSG_E_ARG, "csg_render: scene is not on a device (call csg_scene_upload)");
   int dev = 0;
   hipError_t e = hipGetDevice(&dev);
   if (e != hipSuccess) return FailHip(e, "hipGetDevice");
-  if (dev != s->device) return Fail(CSG_E_ARG, "csg_render: scene was uploaded to another device");
-
   csgray::RenderParams p{};
-  p.sc = s->dev;
-  const csgray::CameraBasis b = csgray::MakeBasis(cam->pos, cam->pitch, cam->yaw, cam->vfov);
-  for (int k = 0; k < 3; k++) {
-    p.pos[k] = b.pos[k];
-    p.forward[k] = b.forward[k];
-    p.right[k] = b.right[k];
-    p.up[k] = b.up[k];
-  }
-  p.tanHalf = b.tanHalf;
-  csgray::NormalizeLight(light->dir, p.light);
-  p.ka = phong->ka; p.kd = phong->kd; p.ks = phong->ks; p.shininess = phong->shininess;
-  for (int k = 0; k < 4; k++) p.bg[k] = phong->bg[k];
-  p.width = width;
-  p.height = height;
-  p.rgba = reinterpret_cast<float4*>(out->rgba);
-  p.depth = out->depth;
-  p.primId = out->prim_id;
-  p.flags = out->flags;
-  p.counters = cfg.count ? opts->counter_out : nullptr;
+  const csg_status js = BuildJob(s, cam, light, phong, width, height, out, opts, dev, 0, &p.job);
+  if (js != CSG_OK) return js;
+  p.numJobs = 0;
+  p.numTiles = p.job.tiles;
+  p.stackSlots = s->dev.opDepth;
   p.tileOrder = opts ? opts->tile_order : nullptr;
-  p.tilesX = (width + 7) / 8;
-  const int64_t tiles = static_cast<int64_t>(p.tilesX) * ((height + 7) / 8);
-  p.numTiles = static_cast<int32_t>(tiles);
+  p.morton = cfg.morton ? 1 : 0;
   csgray::LaunchShape shape;
-  e = csgray::ChooseShape(s->dev, cfg, &shape);
+  e = csgray::ChooseShape(s->dev.opDepth, SceneBytes(s), cfg, &shape);
   if (e != hipSuccess) return FailHip(e, "choosing the launch shape");
   p.totalWaves = static_cast<uint32_t>(shape.grid) * static_cast<uint32_t>(shape.block / 64);
   const hipStream_t hs = static_cast<hipStream_t>(stream);
@@ -386,6 +476,64 @@ csg_status csg_render_ex(const csg_scene* s, const csg_camera* cam, const csg_li
 csg_status csg_render(const csg_scene* s, const csg_camera* cam, const csg_light* light, const csg_phong* phong,
                       int32_t width, int32_t height, const csg_outputs* out, void* stream) {
   return csg_render_ex(s, cam, light, phong, width, height, out, nullptr, stream);
+}
+
+csg_status csg_batch_order(const csg_job* jobs, int32_t njobs, int32_t* order) {
+  if (!jobs || !order || njobs < 0) return Fail(CSG_E_ARG, "csg_batch_order: bad argument");
+  std::vector<std::pair<double, int32_t>> cost(static_cast<size_t>(njobs));
+  for (int32_t i = 0; i < njobs; i++) {
+    const double depth = jobs[i].scene ? jobs[i].scene->dev.opDepth : 0;
+    cost[i] = {-static_cast<double>(jobs[i].width) * jobs[i].height * (1.0 + depth), i};
+  }
+  std::stable_sort(cost.begin(), cost.end());
+  for (int32_t i = 0; i < njobs; i++) order[i] = cost[i].second;
+  return CSG_OK;
+}
+
+csg_status csg_render_batch(const csg_job* jobs, int32_t njobs, const csg_render_opts* opts, void* stream) {
+  if (!jobs || njobs <= 0) return Fail(CSG_E_ARG, "csg_render_batch: no jobs");
+  if (opts && opts->tile_order) return Fail(CSG_E_ARG, "csg_render_batch: tile_order applies to single renders");
+  csgray::LaunchConfig cfg;
+  const csg_status ost = ParseOpts(opts, &cfg);
+  if (ost != CSG_OK) return ost;
+  if (cfg.sceneInLds == 1) return Fail(CSG_E_ARG, "csg_render_batch: scenes of a batch are read from global memory");
+  cfg.sceneInLds = 0;
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return FailHip(e, "hipGetDevice");
+  const hipStream_t hs = static_cast<hipStream_t>(stream);
+  csgray::RenderParams p{};
+  int32_t depth = 1;
+  int64_t base = 0;
+  for (int32_t start = 0; start < njobs; start += csgray::kMaxBatchJobs) {
+    const int32_t n = std::min(njobs - start, csgray::kMaxBatchJobs);
+    std::vector<csgray::Job> host(static_cast<size_t>(n));
+    base = 0;
+    depth = 1;
+    for (int32_t i = 0; i < n; i++) {
+      const csg_job& j = jobs[start + i];
+      const csg_status js = BuildJob(j.scene, &j.camera, &j.light, &j.phong, j.width, j.height, &j.out, opts, dev,
+                                     base, &host[i]);
+      if (js != CSG_OK) return js;
+      base += host[i].tiles;
+      depth = std::max(depth, j.scene->dev.opDepth);
+    }
+    const csg_status bs = UploadJobs(dev, hs, host.data(), host.size(), &p.jobs);
+    if (bs != CSG_OK) return bs;
+    p.numJobs = n;
+    p.numTiles = static_cast<int32_t>(base);
+    p.stackSlots = depth;
+    p.morton = cfg.morton ? 1 : 0;
+    csgray::LaunchShape shape;
+    e = csgray::ChooseShape(depth, 0, cfg, &shape);
+    if (e != hipSuccess) return FailHip(e, "choosing the launch shape");
+    p.totalWaves = static_cast<uint32_t>(shape.grid) * static_cast<uint32_t>(shape.block / 64);
+    const csg_status qs = QueueFor(dev, hs, &p.queue);
+    if (qs != CSG_OK) return qs;
+    e = csgray::LaunchRender(p, cfg, shape, hs);
+    if (e != hipSuccess) return FailHip(e, "batched render launch");
+  }
+  return CSG_OK;
 }
 
 csg_status csg_sync(void* stream) {

@@ -20,8 +20,10 @@ struct DevScene {
   int32_t opDepth;
 };
 
-// Everything one render needs, passed by value as the kernel argument.
-struct RenderParams {
+// One render job: a scene seen by one camera at one resolution, with its
+// output buffers.  A plain render passes one job by value as the kernel
+// argument; a batched render reads a device array of them.
+struct Job {
   DevScene sc;
   float pos[3];
   float forward[3], right[3], up[3];
@@ -30,16 +32,45 @@ struct RenderParams {
   float ka, kd, ks, shininess;
   float bg[4];
   int32_t width, height;
-  int32_t tilesX, numTiles;  // 8x8-pixel tiles, row-major
-  uint32_t totalWaves;       // waves in the grid (queue reset bookkeeping)
-  uint32_t* queue;           // [0] next tile, [1] waves finished; zero between launches
-  const uint32_t* tileOrder; // optional: the order in which tiles are taken (numTiles entries)
+  int32_t tilesX;     // 8x8-pixel tiles per row
+  int32_t tiles;      // tiles of this job (all samples)
+  int32_t tileBase;   // first queue index of this job (batched)
+  int32_t spp;        // rays per pixel; sample s of pixel p is output entry p*spp + s
+  uint32_t seed;      // jitter seed (spp > 1)
+  int32_t pad_;
   float4* rgba;
   float* depth;
   int32_t* primId;
   uint8_t* flags;
-  int32_t* counters;  // 6 per pixel when counting
+  int32_t* counters;  // 6 per ray when counting
 };
+
+// Launch-wide parameters.
+struct RenderParams {
+  Job job;                    // plain: the one job
+  const Job* jobs;            // batched: device array, ascending tileBase
+  int32_t numJobs;            // 0 = plain
+  int32_t numTiles;           // queue length (all jobs)
+  int32_t stackSlots;         // LDS stack slots per lane (max operator depth of the launch)
+  uint32_t totalWaves;        // waves in the grid (queue reset bookkeeping)
+  uint32_t* queue;            // [0] next tile, [1] waves finished; zero between launches
+  const uint32_t* tileOrder;  // optional: order in which queue entries map to tiles (plain)
+  int32_t morton;             // 1: tiles of a job are visited in Morton order
+  int32_t pad_;
+};
+
+// Jitter of sample s of pixel p: a 32-bit integer hash of (seed, p, s)
+// mapped to [0, 1) with 24 bits; identical on host and device.
+__host__ __device__ inline uint32_t JitterHash(uint32_t seed, uint32_t pixel, uint32_t sample, uint32_t axis) {
+  uint32_t h = seed ^ (pixel * 0x9E3779B1u) ^ (sample * 0x85EBCA77u) ^ (axis * 0xC2B2AE3Du);
+  h ^= h >> 16;
+  h *= 0x7FEB352Du;
+  h ^= h >> 15;
+  h *= 0x846CA68Bu;
+  h ^= h >> 16;
+  return h;
+}
+__host__ __device__ inline float JitterUnit(uint32_t h) { return static_cast<float>(h >> 8) * (1.0f / 16777216.0f); }
 
 enum class Mode : int32_t { kExact = 0, kFast = 1 };
 
@@ -48,6 +79,7 @@ struct LaunchConfig {
   bool count = false;
   int32_t block = 0;   // 0 = choose
   int32_t sceneInLds = -1;  // -1 = choose, 0 = global, 1 = LDS
+  bool morton = false;      // visit each job's tiles in Morton order
 };
 
 // Launch geometry chosen for a scene: workgroup size, grid, LDS bytes.
@@ -58,15 +90,21 @@ struct LaunchShape {
   bool sceneInLds = false;
 };
 
-// Picks the launch shape for `sc` on the current device.
-hipError_t ChooseShape(const DevScene& sc, const LaunchConfig& cfg, LaunchShape* shape);
+// Picks the launch shape on the current device for a launch whose scenes
+// have at most `depth` operator levels; `sceneBytes` > 0 allows staging that
+// many bytes of scene in LDS (single-scene launches only).
+hipError_t ChooseShape(int32_t depth, size_t sceneBytes, const LaunchConfig& cfg, LaunchShape* shape);
 
 // Compute units and LDS bytes per workgroup of the current device.
 hipError_t DeviceShapeLimits(int* computeUnits, size_t* ldsPerBlock);
 
-// Enqueues one render on `stream` with the given shape.
+// Enqueues one launch on `stream` with the given shape (p.numJobs == 0:
+// plain render of p.job; otherwise the batch in p.jobs).
 hipError_t LaunchRender(const RenderParams& p, const LaunchConfig& cfg, const LaunchShape& shape,
                         hipStream_t stream);
+
+// Most jobs one batched launch takes (their tile prefix is staged in LDS).
+constexpr int32_t kMaxBatchJobs = 1024;
 
 // Largest operator depth the traversal stack supports.
 int32_t MaxSupportedDepth();

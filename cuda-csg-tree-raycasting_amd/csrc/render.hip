@@ -467,22 +467,24 @@ __device__ __forceinline__ void Normalize3(float& x, float& y, float& z) {  // F
   z = inv * z;
 }
 
-// Ray for pixel (x, y) (RaycastingKernels.cu:11-27, Ray.cuh:12-18).
-__device__ __forceinline__ RayF MakeRay(const RenderParams& p, int x, int y) {
-  const float width = static_cast<float>(p.width), height = static_cast<float>(p.height);
-  const float u = (static_cast<float>(x) + 0.5f) / (width - 1);
-  const float v = (static_cast<float>(y) + 0.5f) / (height - 1);
-  const float nx = (2.0f * u - 1.0f) * (width / height) * p.tanHalf;
-  const float ny = (1.0f - 2.0f * v) * p.tanHalf;
-  float dx = p.right[0] * nx + p.up[0] * ny + p.forward[0];
-  float dy = p.right[1] * nx + p.up[1] * ny + p.forward[1];
-  float dz = p.right[2] * nx + p.up[2] * ny + p.forward[2];
+// Ray for pixel (x, y) (RaycastingKernels.cu:11-27, Ray.cuh:12-18).  The
+// reference samples the pixel at offset 0.5; with jitter the offsets come
+// from JitterHash.
+__device__ __forceinline__ RayF MakeRay(const Job& jb, int x, int y, float ox, float oy) {
+  const float width = static_cast<float>(jb.width), height = static_cast<float>(jb.height);
+  const float u = (static_cast<float>(x) + ox) / (width - 1);
+  const float v = (static_cast<float>(y) + oy) / (height - 1);
+  const float nx = (2.0f * u - 1.0f) * (width / height) * jb.tanHalf;
+  const float ny = (1.0f - 2.0f * v) * jb.tanHalf;
+  float dx = jb.right[0] * nx + jb.up[0] * ny + jb.forward[0];
+  float dy = jb.right[1] * nx + jb.up[1] * ny + jb.forward[1];
+  float dz = jb.right[2] * nx + jb.up[2] * ny + jb.forward[2];
   Normalize3(dx, dy, dz);
   const float invLen = 1.0f / sqrtf(dx * dx + dy * dy + dz * dz);
   RayF r;
-  r.ox = p.pos[0];
-  r.oy = p.pos[1];
-  r.oz = p.pos[2];
+  r.ox = jb.pos[0];
+  r.oy = jb.pos[1];
+  r.oz = jb.pos[2];
   r.dx = dx * invLen;
   r.dy = dy * invLen;
   r.dz = dz * invLen;
@@ -493,11 +495,10 @@ __device__ __forceinline__ RayF MakeRay(const RenderParams& p, int x, int y) {
 }
 
 // Hit details (:183-200, :338-372, :436-457) and Phong (LightningKernel
-// :62-110), writing the pixel's outputs.
-__device__ __forceinline__ void ShadeAndStore(const RenderParams& p, const SceneView& sv, const RayF& r, Hit h,
-                                              size_t pix) {
+// :62-110), writing output entry `pix`.
+__device__ __forceinline__ void ShadeAndStore(const Job& jb, const SceneView& sv, const RayF& r, Hit h, size_t pix) {
   const uint32_t flags = Flags(h.bits);
-  float4 colour = make_float4(p.bg[0], p.bg[1], p.bg[2], p.bg[3]);
+  float4 colour = make_float4(jb.bg[0], jb.bg[1], jb.bg[2], jb.bg[3]);
   float depth = __builtin_inff();
   int32_t id = -1;
   if (!(flags & kMissBit)) {
@@ -539,9 +540,9 @@ __device__ __forceinline__ void ShadeAndStore(const RenderParams& p, const Scene
     if (flags & kFlip) { Nx = -Nx; Ny = -Ny; Nz = -Nz; }
     if (flags & kExit) { Nx = -Nx; Ny = -Ny; Nz = -Nz; }
 
-    const float4 col = p.sc.primC[prim];
-    const float Lx = p.light[0], Ly = p.light[1], Lz = p.light[2];
-    float Vx = p.pos[0] - px, Vy = p.pos[1] - py, Vz = p.pos[2] - pz;
+    const float4 col = jb.sc.primC[prim];
+    const float Lx = jb.light[0], Ly = jb.light[1], Lz = jb.light[2];
+    float Vx = jb.pos[0] - px, Vy = jb.pos[1] - py, Vz = jb.pos[2] - pz;
     Normalize3(Vx, Vy, Vz);
     // reflect(-L, N) (Float3Utils.cuh:39-44): a - 2(a.n)n with n = normalize(N)
     float nnx = Nx, nny = Ny, nnz = Nz;
@@ -550,10 +551,10 @@ __device__ __forceinline__ void ShadeAndStore(const RenderParams& p, const Scene
     const float s2 = 2.0f * (ax * nnx + ay * nny + az * nnz);
     const float Rx = ax - s2 * nnx, Ry = ay - s2 * nny, Rz = az - s2 * nnz;
     const float diff = fmaxf(Nx * Lx + Ny * Ly + Nz * Lz, 0.0f);
-    const float spec = powf(fmaxf(Vx * Rx + Vy * Ry + Vz * Rz, 0.0f), p.shininess);
-    const float kdiff = p.kd * diff * 1.0f;
-    const float kspec = p.ks * spec * 1.0f;
-    const float ka = p.ka * 1.0f;
+    const float spec = powf(fmaxf(Vx * Rx + Vy * Ry + Vz * Rz, 0.0f), jb.shininess);
+    const float kdiff = jb.kd * diff * 1.0f;
+    const float kspec = jb.ks * spec * 1.0f;
+    const float ka = jb.ka * 1.0f;
     const float fr = col.x * (ka + kdiff + kspec);
     const float fg = col.y * (ka + kdiff + kspec);
     const float fb = col.z * (ka + kdiff + kspec);
@@ -562,56 +563,113 @@ __device__ __forceinline__ void ShadeAndStore(const RenderParams& p, const Scene
     depth = t;
     id = static_cast<int32_t>(prim);
   }
-  if (p.rgba) p.rgba[pix] = colour;
-  if (p.depth) p.depth[pix] = depth;
-  if (p.primId) p.primId[pix] = id;
-  if (p.flags) p.flags[pix] = static_cast<uint8_t>(flags);
+  if (jb.rgba) jb.rgba[pix] = colour;
+  if (jb.depth) jb.depth[pix] = depth;
+  if (jb.primId) jb.primId[pix] = id;
+  if (jb.flags) jb.flags[pix] = static_cast<uint8_t>(flags);
 }
 
-// LDS: [stack slab: opDepth x blockDim x 8 B][scene copy when kLds].
-template <bool kFast, bool kCount, bool kLds>
+// Tile coordinates of the t-th tile of a job whose tile grid is tx wide and
+// ty high: row-major, or Morton order over the enclosing power-of-two square
+// with out-of-grid codes skipped by counting (compact Morton rank).
+__device__ __forceinline__ uint32_t Compact1By1(uint32_t v) {
+  v &= 0x55555555u;
+  v = (v ^ (v >> 1)) & 0x33333333u;
+  v = (v ^ (v >> 2)) & 0x0F0F0F0Fu;
+  v = (v ^ (v >> 4)) & 0x00FF00FFu;
+  v = (v ^ (v >> 8)) & 0x0000FFFFu;
+  return v;
+}
+
+// Traces and shades the 64 rays of one tile (one per lane).
+template <bool kFast, bool kCount>
+__device__ __forceinline__ void RenderTile(const Job& jb, const SceneView& sv, uint32_t local, bool morton,
+                                           uint2* slab, int stride, int cap, const uint16_t* table, int lane) {
+  const uint32_t perSample = static_cast<uint32_t>(jb.tiles / jb.spp);
+  const uint32_t sample = local / perSample;
+  uint32_t t = local - sample * perSample;
+  const uint32_t tilesX = static_cast<uint32_t>(jb.tilesX);
+  uint32_t tx, ty;
+  if (morton && tilesX == perSample / tilesX && (tilesX & (tilesX - 1)) == 0) {
+    tx = Compact1By1(t);
+    ty = Compact1By1(t >> 1);
+  } else {
+    tx = t % tilesX;
+    ty = t / tilesX;
+  }
+  const int x = static_cast<int>(tx) * 8 + (lane & 7);
+  const int y = static_cast<int>(ty) * 8 + (lane >> 3);
+  if (x >= jb.width || y >= jb.height) return;
+  const uint32_t pixel = static_cast<uint32_t>(y) * static_cast<uint32_t>(jb.width) + static_cast<uint32_t>(x);
+  float ox = 0.5f, oy = 0.5f;
+  if (jb.spp > 1) {
+    ox = JitterUnit(JitterHash(jb.seed, pixel, sample, 0));
+    oy = JitterUnit(JitterHash(jb.seed, pixel, sample, 1));
+  }
+  const RayF r = MakeRay(jb, x, y, ox, oy);
+  Counts n{0, 0, 0, 0, 0, 0};
+  const Hit h = Trace<kFast, kCount>(sv, r, jb.sc.rootDesc, slab, stride, cap, table, n);
+  const size_t out = static_cast<size_t>(pixel) * static_cast<size_t>(jb.spp) + sample;
+  ShadeAndStore(jb, sv, r, h, out);
+  if (kCount && jb.counters) {
+    int32_t* c = jb.counters + 6 * out;
+    c[0] = n.go; c[1] = n.comp; c[2] = n.box; c[3] = n.sph; c[4] = n.cyl; c[5] = n.cub;
+  }
+}
+
+// Persistent render kernel.  LDS: [stack slab: stackSlots x blockDim x 8 B]
+// then, when kLds, the scene copy (plain launches), or, when batched, the
+// jobs' first queue indices.
+template <bool kFast, bool kCount, bool kLds, bool kBatch>
 __global__ void __launch_bounds__(1024) RenderKernel(RenderParams p) {
   extern __shared__ uint4 ldsBase[];
   __shared__ uint16_t table[27];
   const int tid = threadIdx.x;
-  const int cap = p.sc.opDepth > 0 ? p.sc.opDepth : 1;
+  const int cap = p.stackSlots > 0 ? p.stackSlots : 1;
   uint2* slab = reinterpret_cast<uint2*>(ldsBase);
   if (tid < 27) table[tid] = kActionTable[tid];
+  int32_t* jobBase = reinterpret_cast<int32_t*>(slab + static_cast<size_t>(cap) * blockDim.x);
 
-  SceneView sv{p.sc.ops, p.sc.primA, p.sc.primB};
+  SceneView sv{p.job.sc.ops, p.job.sc.primA, p.job.sc.primB};
   if (kLds) {
     float4* lds = reinterpret_cast<float4*>(slab + static_cast<size_t>(cap) * blockDim.x);
-    const int nOps4 = 4 * p.sc.nOps, nP = p.sc.nPrims;
-    for (int i = tid; i < nOps4; i += blockDim.x) lds[i] = p.sc.ops[i];
+    const int nOps4 = 4 * p.job.sc.nOps, nP = p.job.sc.nPrims;
+    for (int i = tid; i < nOps4; i += blockDim.x) lds[i] = p.job.sc.ops[i];
     for (int i = tid; i < nP; i += blockDim.x) {
-      lds[nOps4 + i] = p.sc.primA[i];
-      lds[nOps4 + nP + i] = p.sc.primB[i];
+      lds[nOps4 + i] = p.job.sc.primA[i];
+      lds[nOps4 + nP + i] = p.job.sc.primB[i];
     }
     sv.ops = lds;
     sv.primA = lds + nOps4;
     sv.primB = lds + nOps4 + nP;
   }
+  if (kBatch) {
+    for (int i = tid; i < p.numJobs; i += blockDim.x) jobBase[i] = p.jobs[i].tileBase;
+  }
   __syncthreads();
 
   const int lane = tid & 63;
   for (;;) {
-    uint32_t tile = 0;
-    if (lane == 0) tile = atomicAdd(p.queue, 1u);
-    tile = __builtin_amdgcn_readfirstlane(tile);
-    if (tile >= static_cast<uint32_t>(p.numTiles)) break;
-    if (p.tileOrder) tile = p.tileOrder[tile];
-    const int x = static_cast<int>(tile % p.tilesX) * 8 + (lane & 7);
-    const int y = static_cast<int>(tile / p.tilesX) * 8 + (lane >> 3);
-    if (x < p.width && y < p.height) {
-      const RayF r = MakeRay(p, x, y);
-      Counts n{0, 0, 0, 0, 0, 0};
-      const Hit h = Trace<kFast, kCount>(sv, r, p.sc.rootDesc, slab + tid, blockDim.x, cap, table, n);
-      const size_t pix = static_cast<size_t>(y) * p.width + x;
-      ShadeAndStore(p, sv, r, h, pix);
-      if (kCount && p.counters) {
-        int32_t* c = p.counters + 6 * pix;
-        c[0] = n.go; c[1] = n.comp; c[2] = n.box; c[3] = n.sph; c[4] = n.cyl; c[5] = n.cub;
+    uint32_t q = 0;
+    if (lane == 0) q = atomicAdd(p.queue, 1u);
+    q = __builtin_amdgcn_readfirstlane(q);
+    if (q >= static_cast<uint32_t>(p.numTiles)) break;
+    if (kBatch) {
+      // the job whose queue range holds q: last tileBase <= q
+      int lo = 0, hi = p.numJobs - 1;
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (static_cast<uint32_t>(jobBase[mid]) <= q) lo = mid;
+        else hi = mid - 1;
       }
+      const int j = __builtin_amdgcn_readfirstlane(lo);
+      const Job& jb = p.jobs[j];
+      const SceneView jsv{jb.sc.ops, jb.sc.primA, jb.sc.primB};
+      RenderTile<kFast, kCount>(jb, jsv, q - static_cast<uint32_t>(jb.tileBase), p.morton != 0, slab + tid,
+                                blockDim.x, cap, table, lane);
+    } else {
+      const uint32_t t = p.tileOrder ? p.tileOrder[q] : q;
+      RenderTile<kFast, kCount>(p.job, sv, t, p.morton != 0, slab + tid, blockDim.x, cap, table, lane);
     }
   }
   // The last wave out returns the queue to zero for the next launch.
@@ -627,9 +685,9 @@ __global__ void __launch_bounds__(1024) RenderKernel(RenderParams p) {
 constexpr int kMaxDepth = 64;
 constexpr size_t kStaticLds = 256;  // the action table and alignment slack
 
-template <bool kFast, bool kCount, bool kLds>
+template <bool kFast, bool kCount, bool kLds, bool kBatch>
 hipError_t Launch(const RenderParams& p, const LaunchShape& s, hipStream_t stream) {
-  auto fn = RenderKernel<kFast, kCount, kLds>;
+  auto fn = RenderKernel<kFast, kCount, kLds, kBatch>;
   if (s.lds > 64 * 1024) {
     const hipError_t e =
         hipFuncSetAttribute(reinterpret_cast<const void*>(fn), hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -654,10 +712,6 @@ DeviceLimits Limits(int dev) {
   return l;
 }
 
-}  // namespace
-
-int32_t MaxSupportedDepth() { return kMaxDepth; }
-
 const DeviceLimits* CurrentLimits() {
   static thread_local int cachedDev = -1;
   static thread_local DeviceLimits lim;
@@ -670,6 +724,10 @@ const DeviceLimits* CurrentLimits() {
   return &lim;
 }
 
+}  // namespace
+
+int32_t MaxSupportedDepth() { return kMaxDepth; }
+
 hipError_t DeviceShapeLimits(int* computeUnits, size_t* ldsPerBlock) {
   const DeviceLimits* l = CurrentLimits();
   if (!l) return hipErrorNoDevice;
@@ -678,20 +736,21 @@ hipError_t DeviceShapeLimits(int* computeUnits, size_t* ldsPerBlock) {
   return hipSuccess;
 }
 
-hipError_t ChooseShape(const DevScene& sc, const LaunchConfig& cfg, LaunchShape* shape) {
+hipError_t ChooseShape(int32_t depthIn, size_t sceneBytes, const LaunchConfig& cfg, LaunchShape* shape) {
   const DeviceLimits* lp = CurrentLimits();
   if (!lp) return hipErrorNoDevice;
   const DeviceLimits& lim = *lp;
   const int cus = lim.cus > 0 ? lim.cus : 256;
   const size_t ldsCap = lim.ldsPerBlock > 0 ? lim.ldsPerBlock : 64 * 1024;
-  const size_t depth = static_cast<size_t>(sc.opDepth > 0 ? sc.opDepth : 1);
-  const size_t sceneBytes = (static_cast<size_t>(sc.nOps) * 4 + static_cast<size_t>(sc.nPrims) * 2) * 16;
+  const size_t depth = static_cast<size_t>(depthIn > 0 ? depthIn : 1);
+  const size_t extra = kStaticLds + (sceneBytes == 0 ? kMaxBatchJobs * 4 : 0);
   LaunchShape s;
   bool lds = false;
   int block = cfg.block > 0 ? cfg.block : 0;
-  if (cfg.sceneInLds != 0) {
-    // the largest workgroup (most waves sharing one scene copy) that fits
-    for (int b : {1024, 512, 256}) {
+  if (cfg.sceneInLds != 0 && sceneBytes > 0) {
+    // the largest workgroup (most waves sharing one scene copy) that fits;
+    // 512 lanes measured best on the cheese frame, so it is tried first
+    for (int b : {512, 1024, 256}) {
       if (block && b != block) continue;
       if (depth * b * 8 + sceneBytes + kStaticLds <= ldsCap) {
         lds = true;
@@ -703,12 +762,12 @@ hipError_t ChooseShape(const DevScene& sc, const LaunchConfig& cfg, LaunchShape*
   if (!lds) {
     if (cfg.sceneInLds == 1) return hipErrorInvalidValue;
     if (!block) block = 256;
-    while (block > 64 && depth * block * 8 + kStaticLds > ldsCap) block /= 2;
-    if (depth * block * 8 + kStaticLds > ldsCap) return hipErrorInvalidValue;
+    while (block > 64 && depth * block * 8 + extra > ldsCap) block /= 2;
+    if (depth * block * 8 + extra > ldsCap) return hipErrorInvalidValue;
   }
   s.block = block;
   s.sceneInLds = lds;
-  s.lds = depth * block * 8 + (lds ? sceneBytes : 0);
+  s.lds = depth * block * 8 + (lds ? sceneBytes : extra - kStaticLds);
   // enough resident waves to fill every CU; extra blocks find the queue empty
   const int blocksPerCu = lds ? 1 : std::max(1, 2048 / block);
   s.grid = cus * blocksPerCu;
@@ -718,12 +777,16 @@ hipError_t ChooseShape(const DevScene& sc, const LaunchConfig& cfg, LaunchShape*
 
 hipError_t LaunchRender(const RenderParams& p, const LaunchConfig& cfg, const LaunchShape& s, hipStream_t stream) {
   const bool fast = cfg.mode == Mode::kFast;
-  if (s.sceneInLds) {
-    if (fast) return cfg.count ? Launch<true, true, true>(p, s, stream) : Launch<true, false, true>(p, s, stream);
-    return cfg.count ? Launch<false, true, true>(p, s, stream) : Launch<false, false, true>(p, s, stream);
+  if (p.numJobs > 0) {
+    if (fast) return cfg.count ? Launch<true, true, false, true>(p, s, stream) : Launch<true, false, false, true>(p, s, stream);
+    return cfg.count ? Launch<false, true, false, true>(p, s, stream) : Launch<false, false, false, true>(p, s, stream);
   }
-  if (fast) return cfg.count ? Launch<true, true, false>(p, s, stream) : Launch<true, false, false>(p, s, stream);
-  return cfg.count ? Launch<false, true, false>(p, s, stream) : Launch<false, false, false>(p, s, stream);
+  if (s.sceneInLds) {
+    if (fast) return cfg.count ? Launch<true, true, true, false>(p, s, stream) : Launch<true, false, true, false>(p, s, stream);
+    return cfg.count ? Launch<false, true, true, false>(p, s, stream) : Launch<false, false, true, false>(p, s, stream);
+  }
+  if (fast) return cfg.count ? Launch<true, true, false, false>(p, s, stream) : Launch<true, false, false, false>(p, s, stream);
+  return cfg.count ? Launch<false, true, false, false>(p, s, stream) : Launch<false, false, false, false>(p, s, stream);
 }
 
 }  // namespace csgray

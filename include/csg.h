@@ -109,6 +109,11 @@ typedef struct csg_render_opts {
   int32_t* counter_out; /* device, 6 int32 per pixel: GoTo, Compute, box, sphere, cylinder, cube */
   const uint32_t* tile_order; /* device, optional: order in which the ceil(w/8) x ceil(h/8) tiles of
                                  8x8 pixels (row-major ids) are taken; a permutation, or NULL */
+  int32_t samples;      /* rays per pixel (0 or 1: the reference's single ray through the pixel centre);
+                           with n > 1 every output buffer holds w*h*n entries, sample s of pixel p at
+                           p*n + s, each ray offset within its pixel by csg_jitter() */
+  uint32_t seed;        /* jitter seed */
+  int32_t tile_walk;    /* 0: row-major tiles; 1: Morton order within a job (square power-of-two grids) */
 } csg_render_opts;
 
 /* --- scene (replaces CSGTree::Parse + the upload in Raycaster::ChangeSize) --- */
@@ -144,6 +149,29 @@ csg_status csg_render_ex(const csg_scene* scene, const csg_camera* cam, const cs
                          const csg_phong* phong, int32_t width, int32_t height, const csg_outputs* out,
                          const csg_render_opts* opts, void* stream);
 csg_status csg_sync(void* stream);
+
+/* Sub-pixel offset in [0,1) of sample `sample` of pixel `pixel` along axis 0
+ * (x) or 1 (y): a 32-bit integer hash of (seed, pixel, sample, axis) with 24
+ * bits kept; the kernels use the same function.  (No reference counterpart:
+ * the reference casts one ray per pixel through its centre.) */
+float csg_jitter(uint32_t seed, uint32_t pixel, uint32_t sample, uint32_t axis);
+
+/* --- batched render (no reference counterpart: the reference renders one
+ * tree per launch).  One persistent launch renders every job; waves take
+ * 8x8 tiles from one queue across all jobs, in the order the jobs are given
+ * (csg_batch_order sorts them heaviest first). --- */
+typedef struct csg_job {
+  const csg_scene* scene;
+  csg_camera camera;
+  csg_light light;
+  csg_phong phong;
+  int32_t width, height;
+  csg_outputs out;
+} csg_job;
+csg_status csg_render_batch(const csg_job* jobs, int32_t njobs, const csg_render_opts* opts, void* stream);
+/* Fills order[njobs] with job indices by descending predicted cost
+ * (pixels x (1 + operator depth)). */
+csg_status csg_batch_order(const csg_job* jobs, int32_t njobs, int32_t* order);
 
 /* Launch geometry csg_render_ex would use for this scene and options on the
  * current device (no reference counterpart; for tests and tuning). */

@@ -820,10 +820,10 @@ struct PixelOut {
 template <class Real>
 static PixelOut render_pixel(const Scene<Real>& s, const Camera<Real>& cam, Real tanHalf, const Real* lightN,
                              const float* phong, int x, int y, int W, int H, int stack_cap, bool flag,
-                             Counters* cnt_out, uint32_t* degen_out) {
+                             Counters* cnt_out, uint32_t* degen_out, float ox = 0.5f, float oy = 0.5f) {
   Real width = (Real)W, height = (Real)H;
-  Real u = ((Real)x + Real(0.5f)) / (g_mutation == 3 ? width : width - 1);
-  Real v = ((Real)y + Real(0.5f)) / (g_mutation == 3 ? height : height - 1);
+  Real u = ((Real)x + Real(ox)) / (g_mutation == 3 ? width : width - 1);
+  Real v = ((Real)y + Real(oy)) / (g_mutation == 3 ? height : height - 1);
   Real nx = (Real(2.0f) * u - Real(1.0f)) * (width / height) * tanHalf;
   Real ny = (Real(1.0f) - Real(2.0f) * v) * tanHalf;
   V3<Real> ro = mk(cam.x, cam.y, cam.z);
@@ -897,6 +897,20 @@ static PixelOut render_pixel(const Scene<Real>& s, const Camera<Real>& cam, Real
   return o;
 }
 
+// Sub-pixel offset for multi-sample rays (a product extension; the
+// reference casts one ray through the pixel centre): 32-bit
+// multiply-xorshift hash of (seed, pixel, sample, axis), top 24 bits / 2^24.
+static float jitter01(uint32_t seed, uint32_t pixel, uint32_t sample, uint32_t axis) {
+  uint32_t v = seed;
+  v ^= pixel * 2654435761u;  // 0x9E3779B1
+  v ^= sample * 2246822519u; // 0x85EBCA77
+  v ^= axis * 3266489917u;   // 0xC2B2AE3D
+  v = (v ^ (v >> 16)) * 2146121005u;   // 0x7FEB352D
+  v = (v ^ (v >> 15)) * 2221713035u;   // 0x846CA68B
+  v ^= v >> 16;
+  return (float)(v >> 8) * (1.0f / 16777216.0f);
+}
+
 struct Handle {
   Tree tree;
   Scene<float> s32;
@@ -904,6 +918,8 @@ struct Handle {
 };
 
 struct RenderArgs {
+  int spp = 1;
+  uint32_t seed = 0;
   const Handle* h;
   const float* cam6;
   const float* light3;
@@ -930,12 +946,19 @@ template <class Real> static void render_rows(const RenderArgs& a, const Scene<R
   int rowidx = 0;
   for (int y = a.row_begin; y < a.row_end; y += a.row_step, rowidx++) {
     if (rowidx % tstep != t0) continue;
-    for (int x = 0; x < a.W; x++) {
-      size_t i = (size_t)y * a.W + x;
+    for (int x = 0; x < a.W * a.spp; x++) {
+      const int px = x / a.spp, smp = x % a.spp;
+      size_t i = (size_t)y * a.W * a.spp + x;
       Counters cnt;
       uint32_t dg = 0;
-      PixelOut o = render_pixel<Real>(s, cam, tanHalf, lightN, a.phong8, x, y, a.W, a.H, a.stack_cap, flag,
-                                      &cnt, &dg);
+      float ox = 0.5f, oy = 0.5f;
+      if (a.spp > 1) {
+        const uint32_t pixel = (uint32_t)y * (uint32_t)a.W + (uint32_t)px;
+        ox = jitter01(a.seed, pixel, (uint32_t)smp, 0);
+        oy = jitter01(a.seed, pixel, (uint32_t)smp, 1);
+      }
+      PixelOut o = render_pixel<Real>(s, cam, tanHalf, lightN, a.phong8, px, y, a.W, a.H, a.stack_cap, flag,
+                                      &cnt, &dg, ox, oy);
       if (a.rgba) std::memcpy(a.rgba + 4 * i, o.rgba, sizeof(o.rgba));
       if (a.depth) a.depth[i] = o.depth;
       if (a.id) a.id[i] = o.id;
@@ -1170,11 +1193,34 @@ int oracle_render(const void* hp, const float* cam6, const float* light3, const 
                   int real_bits, int stack_cap, int nthreads, int row_begin, int row_end, int row_step,
                   float* rgba, float* depth, int32_t* id, uint8_t* flags, int32_t* counters, uint8_t* degen) {
   if (!hp || W < 2 || H < 2 || row_step < 1 || row_begin < 0 || row_end > H || stack_cap < 1) return 1;
-  RenderArgs a{(const Handle*)hp, cam6, light3, phong8, W, H, real_bits, stack_cap, nthreads,
+  RenderArgs a{1, 0u, (const Handle*)hp, cam6, light3, phong8, W, H, real_bits, stack_cap, nthreads,
                row_begin, row_end, row_step, rgba, depth, id, flags, counters, degen, nullptr};
   if (real_bits == 64) render_all<double>(a, a.h->s64);
   else render_all<float>(a, a.h->s32);
   return 0;
+}
+
+// Multi-sample variant: spp rays per pixel with jittered offsets; output
+// arrays hold W*H*spp entries, sample s of pixel p at p*spp + s.
+int oracle_render_samples(const void* hp, const float* cam6, const float* light3, const float* phong8, int W, int H,
+                          int spp, uint32_t seed, int real_bits, int nthreads, int row_begin, int row_end,
+                          int row_step, float* rgba, float* depth, int32_t* id, uint8_t* flags, int32_t* counters,
+                          uint8_t* degen) {
+  if (!hp || W < 2 || H < 2 || spp < 1 || row_step < 1 || row_begin < 0 || row_end > H) return 1;
+  RenderArgs a{};
+  a.spp = spp;
+  a.seed = seed;
+  a.h = (const Handle*)hp; a.cam6 = cam6; a.light3 = light3; a.phong8 = phong8; a.W = W; a.H = H;
+  a.real_bits = real_bits; a.stack_cap = 32; a.nthreads = nthreads; a.row_begin = row_begin; a.row_end = row_end;
+  a.row_step = row_step; a.rgba = rgba; a.depth = depth; a.id = id; a.flags = flags; a.counters = counters;
+  a.degen = degen;
+  if (real_bits == 64) render_all<double>(a, a.h->s64);
+  else render_all<float>(a, a.h->s32);
+  return 0;
+}
+
+float oracle_jitter(uint32_t seed, uint32_t pixel, uint32_t sample, uint32_t axis) {
+  return jitter01(seed, pixel, sample, axis);
 }
 
 // Interval classifier (FP64) on a W x H image; per pixel: status (1 hit,

@@ -50,6 +50,12 @@ def lib():
         L.oracle_classify.restype = ctypes.c_int
         L.oracle_classify.argtypes = [_P, _F, ctypes.c_int, ctypes.c_int, ctypes.c_double, _P, _P, _P, _P]
         L.oracle_set_mutation.argtypes = [ctypes.c_int]
+        L.oracle_render_samples.restype = ctypes.c_int
+        L.oracle_render_samples.argtypes = [_P, _F, _F, _F, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint32,
+                                            ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                            _P, _P, _P, _P, _P, _P]
+        L.oracle_jitter.restype = ctypes.c_float
+        L.oracle_jitter.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32]
         _LIB = L
     return _LIB
 
@@ -125,6 +131,29 @@ class OracleScene:
             raise ValueError("oracle_render: bad arguments")
         return out
 
+    def render_samples(self, cam6, light3, phong8, width, height, spp, seed, bits=32, threads=0, rows=None,
+                       counters=False):
+        """spp jittered rays per pixel; arrays shaped (H, W*spp, ...): sample s of pixel x at column x*spp+s."""
+        if threads <= 0:
+            threads = min(os.cpu_count() or 1, 16)
+        r0, r1, rs = rows if rows is not None else (0, height, 1)
+        w = width * spp
+        out = {
+            "rgba": np.zeros((height, w, 4), np.float32),
+            "depth": np.full((height, w), np.nan, np.float32),
+            "prim_id": np.full((height, w), -2, np.int32),
+            "flags": np.zeros((height, w), np.uint8),
+        }
+        if counters:
+            out["counters"] = np.zeros((height, w, 6), np.int32)
+        st = lib().oracle_render_samples(
+            self.h, _f32(cam6), _f32(light3), _f32(phong8), width, height, spp, seed, bits, threads, r0, r1, rs,
+            out["rgba"].ctypes.data, out["depth"].ctypes.data, out["prim_id"].ctypes.data, out["flags"].ctypes.data,
+            out["counters"].ctypes.data if counters else None, None)
+        if st != 0:
+            raise ValueError("oracle_render_samples: bad arguments")
+        return out
+
     def classify(self, cam6, width, height, sep=1e-6):
         st = np.zeros((height, width), np.int8)
         t = np.zeros((height, width), np.float64)
@@ -134,6 +163,10 @@ class OracleScene:
                                  pid.ctypes.data, en.ctypes.data) != 0:
             raise ValueError("oracle_classify: bad arguments")
         return {"status": st, "t": t, "prim_id": pid, "enter": en}
+
+
+def jitter(seed, pixel, sample, axis):
+    return float(lib().oracle_jitter(seed, pixel, sample, axis))
 
 
 def set_mutation(m: int):

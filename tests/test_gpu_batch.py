@@ -1,0 +1,141 @@
+"""Batched persistent launch and multi-sample rays on the GPU.
+
+Every job of a batch must be bit-identical (all four outputs) to the same job
+rendered alone, whatever else shares the launch; exact-mode jobs are also
+bit-identical to the FP32 oracle (hit id, depth, flags, event counters).
+Jittered multi-sample rays are checked against the oracle the same way.
+"""
+import numpy as np
+import pytest
+import torch
+
+import csgray as C
+import oracle as O
+from common import setups
+from conftest import scene_names, scene_text
+from csgray import scenes as S
+
+pytestmark = pytest.mark.gpu
+
+
+def _np(d):
+    return {k: v.cpu().numpy() for k, v in d.items()}
+
+
+def mixed_jobs(n=24, seed=7):
+    """Reference scenes and random trees with orbit cameras and mixed sizes."""
+    rng = np.random.default_rng(seed)
+    names = scene_names()
+    texts = [scene_text(names[i % len(names)]) for i in range(n // 2)]
+    texts += [S.random_tree(rng) for _ in range(n - n // 2)]
+    out = []
+    for t in texts:
+        eye = S.orbit_eye(rng, 5.0, 80.0)
+        w = int(rng.integers(2, 200))
+        h = int(rng.integers(2, 120))
+        out.append((t, eye, w, h))
+    return out
+
+
+def test_batch_jobs_equal_individual_renders_and_oracle(gpu):
+    print("\n[case] batch of 24 mixed jobs, exact mode", flush=True)
+    light, ph = C.light(S.LIGHT_TOWARD), C.phong(*S.PHONG)
+    specs, alone, scenes = [], [], []
+    for text, eye, w, h in mixed_jobs():
+        sc = C.Scene(text)
+        cam = C.camera_look_at(eye, (0, 0, 0), 60)
+        scenes.append((sc, cam, text))
+        alone.append(_np(sc.render(cam, light, ph, w, h, mode=C.MODE_EXACT, counters=True)))
+        specs.append((sc, cam, light, ph, w, h, C.alloc_outputs(w, h, counters=True)))
+    torch.cuda.synchronize()
+    jobs = C.make_jobs(specs)
+    opts = C.RenderOpts(C.MODE_EXACT, 0, C.PLACE_GLOBAL, 0, None, None, 1, 0, 0)
+    # counters are per job; the batch API takes them through each job's own buffer only when
+    # the opts buffer is set, so count through the plain path and compare the four outputs here
+    C.render_batch(jobs, mode=C.MODE_EXACT)
+    torch.cuda.synchronize()
+    for (sc, cam, text), spec, a in zip(scenes, specs, alone):
+        b = _np(spec[6])
+        for k in ("rgba", "depth", "prim_id", "flags"):
+            assert np.array_equal(a[k].view(np.uint8), b[k].view(np.uint8)), k
+        w, h = spec[4], spec[5]
+        o = O.OracleScene(text).render(C.cam_tuple(cam), S.LIGHT_TOWARD, O.phong8(*S.PHONG), w, h, bits=32,
+                                       counters=True)
+        assert np.array_equal(a["prim_id"], o["prim_id"])
+        assert np.array_equal(a["depth"].view(np.uint32), o["depth"].view(np.uint32))
+        assert np.array_equal(a["flags"], o["flags"])
+        assert np.array_equal(a["counters"], o["counters"])
+
+
+def test_batch_fast_mode_equals_individual_fast_renders(gpu):
+    light, ph = C.light(S.LIGHT_TOWARD), C.phong(*S.PHONG)
+    specs, alone = [], []
+    for text, eye, w, h in mixed_jobs(40, seed=11):
+        sc = C.Scene(text)
+        cam = C.camera_look_at(eye, (0, 0, 0), 60)
+        alone.append(_np(sc.render(cam, light, ph, w, h)))
+        specs.append((sc, cam, light, ph, w, h, C.alloc_outputs(w, h)))
+    jobs = C.make_jobs(specs)
+    order = C.batch_order(jobs)
+    jobs_sorted = C.make_jobs([specs[i] for i in order])
+    C.render_batch(jobs_sorted, walk=1)
+    torch.cuda.synchronize()
+    for spec, a in zip(specs, alone):
+        b = _np(spec[6])
+        for k in a:
+            assert np.array_equal(a[k].view(np.uint8), b[k].view(np.uint8)), k
+
+
+def test_batch_order_is_heaviest_first(gpu):
+    light, ph = C.light(S.LIGHT_TOWARD), C.phong(*S.PHONG)
+    scs = [C.Scene(scene_text(n)) for n in ("test1.txt", "testCheese512.txt", "test5.txt")]
+    sizes = [(64, 64), (64, 64), (256, 256)]
+    specs = [(sc, C.camera_default(), light, ph, w, h, C.alloc_outputs(w, h)) for sc, (w, h) in zip(scs, sizes)]
+    assert C.batch_order(C.make_jobs(specs)) == [2, 1, 0]
+
+
+def test_more_jobs_than_one_launch_takes(gpu):
+    light, ph = C.light(S.LIGHT_TOWARD), C.phong(*S.PHONG)
+    sc = C.Scene(scene_text("testWikipedia.txt"))
+    rng = np.random.default_rng(3)
+    specs = []
+    for i in range(1100):
+        cam = C.camera_look_at(S.orbit_eye(rng, 5.0, 80.0), (0, 0, 0), 60)
+        specs.append((sc, cam, light, ph, 9, 7, C.alloc_outputs(9, 7)))
+    C.render_batch(C.make_jobs(specs))
+    torch.cuda.synchronize()
+    for i in (0, 1023, 1024, 1099):
+        sc_, cam, l_, p_, w, h, outs = specs[i]
+        a = _np(sc.render(cam, light, ph, w, h))
+        torch.cuda.synchronize()
+        b = _np(outs)
+        assert np.array_equal(a["prim_id"], b["prim_id"]) and np.array_equal(a["depth"], b["depth"])
+
+
+@pytest.mark.parametrize("name", ["testWikipedia.txt", "testCylinderSphere.txt", "testCheese256.txt",
+                                  "testSingleCube.txt"])
+def test_jittered_samples_match_oracle(gpu, name):
+    print(f"\n[case] {name} spp=4", flush=True)
+    sname, cam, light, phong, cam6, l3, ph8 = setups()[1]
+    text = scene_text(name)
+    w, h, spp, seed = 96, 54, 4, 1234
+    s = C.Scene(text)
+    g = _np(s.render(cam, light, phong, w, h, mode=C.MODE_EXACT, samples=spp, seed=seed, counters=True))
+    torch.cuda.synchronize()
+    o = O.OracleScene(text).render_samples(cam6, l3, ph8, w, h, spp, seed, bits=32, counters=True)
+    assert np.array_equal(g["prim_id"], o["prim_id"])
+    assert np.array_equal(g["depth"].view(np.uint32), o["depth"].view(np.uint32))
+    assert np.array_equal(g["flags"], o["flags"])
+    assert np.array_equal(g["counters"], o["counters"])
+    assert np.abs(g["rgba"] - o["rgba"]).max() <= 2e-6
+
+
+def test_morton_walk_changes_nothing(gpu):
+    sname, cam, light, phong, *_ = setups()[1]
+    s = C.Scene(scene_text("testCheese512.txt"))
+    for (w, h) in ((256, 256), (200, 130)):
+        a = _np(s.render(cam, light, phong, w, h, walk=0))
+        b = _np(s.render(cam, light, phong, w, h, walk=1))
+        torch.cuda.synchronize()
+        for k in a:
+            assert np.array_equal(a[k].view(np.uint8), b[k].view(np.uint8)), k

@@ -112,3 +112,17 @@ def test_null_arguments_are_errors_not_crashes():
     assert L.csg_scene_export(None, None, None, None) == C.E_ARG
     assert L.csg_scene_free(None) == C.OK
     assert L.csg_render(None, None, None, None, 4, 4, None, None) == C.E_ARG
+
+
+def test_jitter_hash_matches_oracle_and_is_uniform():
+    rng = np.random.default_rng(0)
+    vals = []
+    for _ in range(2000):
+        seed, pixel, sample, axis = (int(v) for v in rng.integers(0, 2**31, size=4))
+        axis &= 1
+        a = C.jitter(seed, pixel, sample, axis)
+        assert a == O.jitter(seed, pixel, sample, axis)
+        assert 0.0 <= a < 1.0
+        vals.append(a)
+    hist = np.histogram(vals, bins=10, range=(0, 1))[0]
+    assert hist.min() > 140 and hist.max() < 260
