@@ -54,6 +54,109 @@ def cpu_baseline(text, cam6, light3, phong8, w, h, row_step):
                       f"{threads} threads, {dt:.1f} s"}
 
 
+def _time_ms(torch, fn, reps):
+    fn()
+    torch.cuda.synchronize()
+    a = torch.cuda.Event(enable_timing=True)
+    b = torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(reps):
+        fn()
+    b.record()
+    torch.cuda.synchronize()
+    return a.elapsed_time(b) / reps
+
+
+def bench_batched(torch, C, S, reps):
+    """The 256-tree mix: per-scene launches vs one batched launch vs the
+    captured (hipGraph) batch after autotuning."""
+    mix = S.batch_mix(256, seed=256)
+    light, ph = C.light(S.LIGHT_TOWARD), C.phong(*S.PHONG)
+    specs = []
+    for text, eye, size in mix:
+        sc = C.Scene(text)
+        specs.append((sc, C.camera_look_at(eye, (0, 0, 0), 60), light, ph, size, size, C.alloc_outputs(size, size)))
+    rays = sum(sp[4] * sp[5] for sp in specs)
+
+    def plain():
+        for sc, cam, l, p, w, h, o in specs:
+            sc.render(cam, l, p, w, h, outputs=o)
+
+    jobs = C.make_jobs(specs)
+    plain_ms = _time_ms(torch, plain, reps)
+    batch_ms = _time_ms(torch, lambda: C.render_batch(jobs), reps)
+    batch = C.Batch(jobs)
+    tune = batch.autotune(reps=2)
+    graph_ms = _time_ms(torch, batch.launch, reps)
+    batch.close()
+    return {"workload": "256 random trees (2-1024 leaves, balanced, ops uniform), 128^2..1024^2, orbit cameras",
+            "rays": rays, "plain_per_scene_mrays_s": round(rays / plain_ms / 1e3, 1),
+            "batched_mrays_s": round(rays / batch_ms / 1e3, 1), "graph_mrays_s": round(rays / graph_ms / 1e3, 1),
+            "batched_vs_plain": round(plain_ms / min(batch_ms, graph_ms), 2),
+            "autotune": {"block": tune.block, "tile_walk": tune.tile_walk, "heaviest_first": tune.heaviest_first},
+            "ms": {"plain": round(plain_ms, 2), "batched": round(batch_ms, 2), "graph": round(graph_ms, 2)}}
+
+
+def bench_serving(C, S, lambdas, seconds):
+    """Poisson arrivals of (tree, orbit camera, size) requests against 64
+    resident trees; latency = submit -> result ready."""
+    import threading
+
+    import numpy as np
+    light, ph = C.light(S.LIGHT_TOWARD), C.phong(*S.PHONG)
+    sizes = [(256, 256), (512, 512), (1024, 1024), (1920, 1080)]
+    out = []
+    for lam in lambdas:
+        server = C.Server(budget_bytes=16 << 30, streams=3, max_batch_jobs=256)
+        mix = S.batch_mix(64, seed=64)
+        scenes = [C.Scene(t) for t, _, _ in mix]
+        ids = [server.add_scene(s) for s in scenes]
+        rng = np.random.default_rng(int(lam))
+        n = max(1, int(lam * seconds))
+        gaps = rng.exponential(1.0 / lam, size=n)
+        arrivals = np.cumsum(gaps)
+        reqs = []
+        for _ in range(n):
+            k = int(rng.integers(0, len(ids)))
+            w, h = sizes[int(rng.integers(0, len(sizes)))]
+            reqs.append((ids[k], C.camera_look_at(S.orbit_eye(rng, 4.0, 85.0), (0, 0, 0), 60), w, h))
+        tickets = []
+        stats = []
+        lock = threading.Lock()
+        ready = threading.Semaphore(0)
+
+        def reaper():
+            for i in range(n):
+                ready.acquire()
+                with lock:
+                    t = tickets[i]
+                st = server.wait(t)
+                stats.append((st.submit_s, st.done_s, st.status))
+                server.release(t)
+
+        th = threading.Thread(target=reaper)
+        th.start()
+        t0 = time.perf_counter()
+        for i, (sid, cam, w, h) in enumerate(reqs):
+            while time.perf_counter() - t0 < arrivals[i]:
+                pass
+            t = server.submit(sid, cam, light, ph, w, h)
+            with lock:
+                tickets.append(t)
+            ready.release()
+        th.join()
+        lat = np.array([(d - s) * 1e3 for s, d, _ in stats])
+        span = max(d for _, d, _ in stats) - min(s for s, _, _ in stats)
+        rays = sum(w * h for _, _, w, h in reqs)
+        ss = server.stats()
+        server.close()
+        out.append({"lambda_per_s": lam, "requests": n, "p50_ms": round(float(np.percentile(lat, 50)), 2),
+                    "p99_ms": round(float(np.percentile(lat, 99)), 2), "mrays_s": round(rays / span / 1e6, 1),
+                    "offered_mrays_s": round(rays / arrivals[-1] / 1e6, 1), "batches": ss.batches,
+                    "failed": int(sum(1 for *_, st in stats if st != 0))})
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -61,7 +164,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--mode", choices=("fast", "exact"), default="fast")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-row-step", type=int, default=16)
+    ap.add_argument("--cpu-row-step", type=int, default=1)
+    ap.add_argument("--no-extra", action="store_true", help="skip the batched and serving legs")
+    ap.add_argument("--serve-seconds", type=float, default=3.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
@@ -125,6 +230,11 @@ def main():
                     "flops_per_ray": round(fpr, 1), "kernel": "RenderKernel<fast>",
                     "avg_kernel_ms": round(avg_kernel_ms, 4)}
 
+    extra = {}
+    if not args.no_extra:
+        extra["batched"] = bench_batched(torch, C, S, reps=3)
+        extra["serving"] = bench_serving(C, S, (200, 1000, 5000), args.serve_seconds)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(text, C.cam_tuple(cam), S.LIGHT_TOWARD, S.PHONG + S.BACKGROUND, w, h,
@@ -142,6 +252,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
+    line.update(extra)
     print(json.dumps(line))
 
 

@@ -68,6 +68,32 @@ class LaunchInfo(ctypes.Structure):
                 ("scene_in_lds", ctypes.c_int32), ("compute_units", ctypes.c_int32), ("lds_limit", ctypes.c_int64)]
 
 
+class BatchTuning(ctypes.Structure):
+    _fields_ = [("block", ctypes.c_int32), ("tile_walk", ctypes.c_int32), ("heaviest_first", ctypes.c_int32),
+                ("ms", ctypes.c_float)]
+
+
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("budget_bytes", ctypes.c_int64), ("streams", ctypes.c_int32), ("max_batch_jobs", ctypes.c_int32),
+                ("mode", ctypes.c_int32)]
+
+
+class Request(ctypes.Structure):
+    _fields_ = [("scene_id", ctypes.c_int32), ("camera", Camera), ("light", Light), ("phong", Phong),
+                ("width", ctypes.c_int32), ("height", ctypes.c_int32)]
+
+
+class RequestStats(ctypes.Structure):
+    _fields_ = [("submit_s", ctypes.c_double), ("start_s", ctypes.c_double), ("done_s", ctypes.c_double),
+                ("batch_jobs", ctypes.c_int32), ("status", ctypes.c_int32)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("submitted", ctypes.c_int64), ("completed", ctypes.c_int64), ("rejected", ctypes.c_int64),
+                ("batches", ctypes.c_int64), ("rays_done", ctypes.c_int64), ("bytes_in_use", ctypes.c_int64),
+                ("peak_bytes_in_use", ctypes.c_int64)]
+
+
 class CsgError(RuntimeError):
     def __init__(self, status, msg):
         super().__init__(f"{STATUS_NAMES.get(status, status)}: {msg}")
@@ -103,6 +129,21 @@ _SIGNATURES = [
     ("csg_jitter", ctypes.c_float, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32]),
     ("csg_render_batch", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(RenderOpts), _P]),
     ("csg_batch_order", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p]),
+    ("csg_batch_create", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(RenderOpts),
+                                        ctypes.POINTER(_P)]),
+    ("csg_batch_update", ctypes.c_int, [_P, ctypes.c_void_p, ctypes.c_int32, _P]),
+    ("csg_batch_launch", ctypes.c_int, [_P, _P]),
+    ("csg_batch_autotune", ctypes.c_int, [_P, ctypes.c_int32, _P, ctypes.POINTER(BatchTuning)]),
+    ("csg_batch_free", ctypes.c_int, [_P]),
+    ("csg_server_create", ctypes.c_int, [ctypes.POINTER(ServerConfig), ctypes.POINTER(_P)]),
+    ("csg_server_add_scene", ctypes.c_int, [_P, _P, ctypes.POINTER(ctypes.c_int32)]),
+    ("csg_server_submit", ctypes.c_int, [_P, ctypes.POINTER(Request), ctypes.POINTER(ctypes.c_int64)]),
+    ("csg_server_poll", ctypes.c_int, [_P, ctypes.c_int64, ctypes.POINTER(ctypes.c_int32)]),
+    ("csg_server_wait", ctypes.c_int, [_P, ctypes.c_int64, ctypes.POINTER(Outputs), ctypes.POINTER(RequestStats)]),
+    ("csg_server_copy", ctypes.c_int, [_P, ctypes.c_int64, ctypes.POINTER(Outputs), _P]),
+    ("csg_server_release", ctypes.c_int, [_P, ctypes.c_int64]),
+    ("csg_server_stats_get", ctypes.c_int, [_P, ctypes.POINTER(ServerStats)]),
+    ("csg_server_destroy", ctypes.c_int, [_P]),
     ("csg_last_error", ctypes.c_char_p, []),
     ("csg_version", ctypes.c_char_p, []),
 ]
@@ -304,6 +345,101 @@ def render_batch(jobs, mode=MODE_FAST, block=0, walk=0, samples=1, seed=0, strea
     opts = RenderOpts(mode, block, PLACE_GLOBAL, 0, None, None, samples, seed, walk)
     s = stream if stream is not None else torch.cuda.current_stream().cuda_stream
     _check(lib().csg_render_batch(jobs, len(jobs), ctypes.byref(opts), s))
+
+
+class Batch:
+    """A captured batch (csg_batch*): jobs fixed at creation, graph replayed by launch()."""
+
+    def __init__(self, jobs, mode=MODE_FAST, block=0, walk=0, samples=1, seed=0):
+        opts = RenderOpts(mode, block, PLACE_GLOBAL, 0, None, None, samples, seed, walk)
+        h = _P()
+        _check(lib().csg_batch_create(jobs, len(jobs), ctypes.byref(opts), ctypes.byref(h)))
+        self.handle = h
+        self.jobs = jobs
+
+    def _stream(self, stream):
+        import torch
+        return stream if stream is not None else torch.cuda.current_stream().cuda_stream
+
+    def launch(self, stream=None):
+        _check(lib().csg_batch_launch(self.handle, self._stream(stream)))
+
+    def update(self, jobs, stream=None):
+        _check(lib().csg_batch_update(self.handle, jobs, len(jobs), self._stream(stream)))
+        self.jobs = jobs
+
+    def autotune(self, reps=2, stream=None) -> BatchTuning:
+        t = BatchTuning()
+        _check(lib().csg_batch_autotune(self.handle, int(reps), self._stream(stream), ctypes.byref(t)))
+        return t
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().csg_batch_free(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """csg_server*: resident scenes, request queue, continuous batching."""
+
+    def __init__(self, budget_bytes=0, streams=0, max_batch_jobs=0, mode=MODE_FAST):
+        h = _P()
+        cfg = ServerConfig(int(budget_bytes), int(streams), int(max_batch_jobs), int(mode))
+        _check(lib().csg_server_create(ctypes.byref(cfg), ctypes.byref(h)))
+        self.handle = h
+        self._scenes = []
+
+    def add_scene(self, scene):
+        i = ctypes.c_int32()
+        _check(lib().csg_server_add_scene(self.handle, scene.handle, ctypes.byref(i)))
+        self._scenes.append(scene)
+        return i.value
+
+    def submit(self, scene_id, cam, light_, phong_, width, height):
+        t = ctypes.c_int64()
+        r = Request(int(scene_id), cam, light_, phong_, int(width), int(height))
+        _check(lib().csg_server_submit(self.handle, ctypes.byref(r), ctypes.byref(t)))
+        return t.value
+
+    def poll(self, ticket):
+        d = ctypes.c_int32()
+        _check(lib().csg_server_poll(self.handle, int(ticket), ctypes.byref(d)))
+        return bool(d.value)
+
+    def wait(self, ticket):
+        st = RequestStats()
+        _check(lib().csg_server_wait(self.handle, int(ticket), None, ctypes.byref(st)))
+        return st
+
+    def copy(self, ticket, outputs, stream=None):
+        import torch
+        s = stream if stream is not None else torch.cuda.current_stream().cuda_stream
+        _check(lib().csg_server_copy(self.handle, int(ticket), ctypes.byref(_outputs_struct(outputs)), s))
+
+    def release(self, ticket):
+        _check(lib().csg_server_release(self.handle, int(ticket)))
+
+    def stats(self):
+        st = ServerStats()
+        _check(lib().csg_server_stats_get(self.handle, ctypes.byref(st)))
+        return st
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().csg_server_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 @dataclass

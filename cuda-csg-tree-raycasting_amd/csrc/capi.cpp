@@ -14,18 +14,11 @@
 
 #include "camera.hpp"
 #include "device.hpp"
+#include "runtime.hpp"
 #include "scene.hpp"
 
-struct csg_scene {
-  csgray::CSGTree tree;
-  csgray::FlatScene flat;
-  int device = 0;
-  void* dmem = nullptr;
-  size_t bytes = 0;
-  csgray::DevScene dev{};
-};
-
-namespace {
+namespace csgray {
+namespace rt {
 
 thread_local std::string g_lastError;
 
@@ -106,7 +99,10 @@ csg_status QueueFor(int dev, hipStream_t stream, uint32_t** out) {
   return CSG_OK;
 }
 
-}  // namespace
+}  // namespace rt
+}  // namespace csgray
+
+using namespace csgray::rt;
 
 extern "C" {
 
@@ -291,7 +287,10 @@ void csg_phong_default(csg_phong* ph) {  // RaycastingKernels.cu:71-74, 109
   ph->bg[0] = 0.08f; ph->bg[1] = 0.08f; ph->bg[2] = 0.11f; ph->bg[3] = 1.0f;
 }
 
-namespace {
+}  // extern "C"
+
+namespace csgray {
+namespace rt {
 csg_status ParseOpts(const csg_render_opts* opts, csgray::LaunchConfig* cfg) {
   if (!opts) return CSG_OK;
   if (opts->mode != CSG_MODE_EXACT && opts->mode != CSG_MODE_FAST) return Fail(CSG_E_ARG, "csg_render: unknown mode");
@@ -415,7 +414,10 @@ csg_status UploadJobs(int dev, hipStream_t stream, const csgray::Job* jobs, size
   *out = static_cast<const csgray::Job*>(t.dev);
   return CSG_OK;
 }
-}  // namespace
+}  // namespace rt
+}  // namespace csgray
+
+extern "C" {
 
 csg_status csg_render_plan(const csg_scene* s, const csg_render_opts* opts, csg_launch_info* info) {
   if (!s || !info) return Fail(CSG_E_ARG, "csg_render_plan: NULL argument");

@@ -1,0 +1,46 @@
+// Internals shared by the C ABI's translation units: the scene handle,
+// error reporting, option parsing, job construction and per-stream device
+// workspaces.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "csg.h"
+#include "device.hpp"
+#include "scene.hpp"
+
+struct csg_scene {
+  csgray::CSGTree tree;
+  csgray::FlatScene flat;
+  int device = 0;
+  void* dmem = nullptr;
+  size_t bytes = 0;
+  csgray::DevScene dev{};
+};
+
+namespace csgray {
+namespace rt {
+
+// Records `msg` as the calling thread's last error and returns `s`.
+csg_status Fail(csg_status s, const std::string& msg);
+csg_status FailHip(hipError_t e, const char* what);
+
+csg_status ParseOpts(const csg_render_opts* opts, LaunchConfig* cfg);
+size_t SceneBytes(const csg_scene* s);
+
+// Validates one job and fills its device descriptor; `tileBase` is its first
+// queue index within the launch.
+csg_status BuildJob(const csg_scene* s, const csg_camera* cam, const csg_light* light, const csg_phong* phong,
+                    int32_t width, int32_t height, const csg_outputs* out, const csg_render_opts* opts, int dev,
+                    int64_t tileBase, Job* jb);
+
+// The tile queue (two words, zero between launches) of (device, stream).
+csg_status QueueFor(int dev, hipStream_t stream, uint32_t** out);
+
+// Copies a job table to the (device, stream) table buffer in stream order.
+csg_status UploadJobs(int dev, hipStream_t stream, const Job* jobs, size_t n, const Job** out);
+
+}  // namespace rt
+}  // namespace csgray

@@ -173,6 +173,29 @@ csg_status csg_render_batch(const csg_job* jobs, int32_t njobs, const csg_render
  * (pixels x (1 + operator depth)). */
 csg_status csg_batch_order(const csg_job* jobs, int32_t njobs, int32_t* order);
 
+/* --- captured batches (no reference counterpart): the batch's job tables and
+ * tile queues live in its own device buffers and its launches are captured
+ * once in a hipGraph; the steady state is csg_batch_launch. --- */
+typedef struct csg_batch csg_batch;
+csg_status csg_batch_create(const csg_job* jobs, int32_t njobs, const csg_render_opts* opts, csg_batch** out);
+/* New jobs (cameras, lights, outputs, sizes; the count may change only
+ * within the captured number of launches): the device job tables are
+ * rewritten in order on `stream`, and when tile counts or stack depth change
+ * the graph's kernel nodes get new parameters. */
+csg_status csg_batch_update(csg_batch* batch, const csg_job* jobs, int32_t njobs, void* stream);
+csg_status csg_batch_launch(csg_batch* batch, void* stream);
+/* Times candidate launch shapes on the batch (workgroup 256/512/1024 lanes x
+ * row-major/Morton tile walk x given/heaviest-first job order, `reps` launches
+ * each), keeps the fastest for later launches and reports it. */
+typedef struct csg_batch_tuning {
+  int32_t block;
+  int32_t tile_walk;
+  int32_t heaviest_first;
+  float ms;  /* mean time of one launch of the batch with this choice */
+} csg_batch_tuning;
+csg_status csg_batch_autotune(csg_batch* batch, int32_t reps, void* stream, csg_batch_tuning* chosen);
+csg_status csg_batch_free(csg_batch* batch);
+
 /* Launch geometry csg_render_ex would use for this scene and options on the
  * current device (no reference counterpart; for tests and tuning). */
 typedef struct csg_launch_info {
@@ -184,6 +207,52 @@ typedef struct csg_launch_info {
   int64_t lds_limit;      /* LDS per workgroup the device allows */
 } csg_launch_info;
 csg_status csg_render_plan(const csg_scene* scene, const csg_render_opts* opts, csg_launch_info* info);
+
+/* --- serving (no reference counterpart: the reference renders one frame at a
+ * time for one window).  A server keeps scenes resident, queues render
+ * requests, admits them in arrival order while their outputs fit a device
+ * memory budget, and keeps up to `streams` batches in flight: requests that
+ * arrive while others run join the next batch on a free stream
+ * (continuous batching).  Outputs are server-owned until released. --- */
+typedef struct csg_server csg_server;
+typedef struct csg_server_config {
+  int64_t budget_bytes;   /* device memory for request outputs; 0 = 16 GiB */
+  int32_t streams;        /* batches in flight; 0 = 3 */
+  int32_t max_batch_jobs; /* requests per batch; 0 = 256 */
+  int32_t mode;           /* csg_mode */
+} csg_server_config;
+typedef struct csg_request {
+  int32_t scene_id;
+  csg_camera camera;
+  csg_light light;
+  csg_phong phong;
+  int32_t width, height;
+} csg_request;
+typedef struct csg_request_stats {
+  double submit_s, start_s, done_s;  /* seconds since the server was created */
+  int32_t batch_jobs;                /* requests in the batch it ran in */
+  int32_t status;                    /* csg_status of its launch (outputs undefined unless CSG_OK) */
+} csg_request_stats;
+typedef struct csg_server_stats {
+  int64_t submitted, completed, rejected, batches, rays_done;
+  int64_t bytes_in_use, peak_bytes_in_use;
+} csg_server_stats;
+csg_status csg_server_create(const csg_server_config* cfg, csg_server** out);
+/* The scene must stay alive and on the server's device while the server runs. */
+csg_status csg_server_add_scene(csg_server* server, const csg_scene* scene, int32_t* scene_id);
+/* CSG_E_BUDGET when the request's outputs alone exceed the budget. */
+csg_status csg_server_submit(csg_server* server, const csg_request* req, int64_t* ticket);
+csg_status csg_server_poll(csg_server* server, int64_t ticket, int32_t* done);
+/* Blocks until the request is done; fills the server-owned device output
+ * pointers (valid until release) and the request's timing. */
+csg_status csg_server_wait(csg_server* server, int64_t ticket, csg_outputs* out, csg_request_stats* stats);
+/* Copies a done request's outputs into caller-owned device buffers (any may
+ * be NULL) on `stream`. */
+csg_status csg_server_copy(csg_server* server, int64_t ticket, const csg_outputs* dst, void* stream);
+csg_status csg_server_release(csg_server* server, int64_t ticket);
+csg_status csg_server_stats_get(csg_server* server, csg_server_stats* stats);
+/* Waits for every request, then frees the server's memory and streams. */
+csg_status csg_server_destroy(csg_server* server);
 
 const char* csg_last_error(void);
 const char* csg_version(void);

@@ -139,3 +139,82 @@ def test_morton_walk_changes_nothing(gpu):
         torch.cuda.synchronize()
         for k in a:
             assert np.array_equal(a[k].view(np.uint8), b[k].view(np.uint8)), k
+
+
+def _specs_for(mix, light, ph, size_override=None):
+    specs = []
+    for text, eye, size in mix:
+        sc = C.Scene(text)
+        cam = C.camera_look_at(eye, (0, 0, 0), 60)
+        w = h = size_override or size
+        specs.append((sc, cam, light, ph, w, h, C.alloc_outputs(w, h)))
+    return specs
+
+
+def _reference(specs):
+    ref = []
+    for sc, cam, l, p, w, h, _ in specs:
+        ref.append(_np(sc.render(cam, l, p, w, h)))
+    torch.cuda.synchronize()
+    return ref
+
+
+def _check(specs, ref):
+    for spec, a in zip(specs, ref):
+        b = _np(spec[6])
+        for k in a:
+            assert np.array_equal(a[k].view(np.uint8), b[k].view(np.uint8)), k
+
+
+def test_captured_batch_replays_match_eager(gpu):
+    print("\n[case] captured batch of 16, 3 replays", flush=True)
+    light, ph = C.light(S.LIGHT_TOWARD), C.phong(*S.PHONG)
+    mix = S.batch_mix(16, seed=99, resolutions=(32, 64, 96))
+    specs = _specs_for(mix, light, ph)
+    ref = _reference(specs)
+    batch = C.Batch(C.make_jobs(specs))
+    for rep in range(3):
+        for spec in specs:
+            for t in spec[6].values():
+                t.fill_(0)
+        torch.cuda.synchronize()
+        batch.launch()
+        torch.cuda.synchronize()
+        _ = torch.ones(1000, device="cuda").sum().item()  # eager work between replays
+        _check(specs, ref)
+
+
+def test_captured_batch_updates_cameras_and_sizes(gpu):
+    light, ph = C.light(S.LIGHT_TOWARD), C.phong(*S.PHONG)
+    mix = S.batch_mix(12, seed=5, resolutions=(40, 72))
+    specs = _specs_for(mix, light, ph)
+    batch = C.Batch(C.make_jobs(specs))
+    rng = np.random.default_rng(8)
+    for rep in range(3):
+        new = []
+        for sc, cam, l, p, w, h, _ in specs:
+            cam2 = C.camera_look_at(S.orbit_eye(rng, 4.0, 80.0), (0, 0, 0), 60)
+            w2, h2 = (w, h) if rep == 0 else (int(rng.integers(8, 90)), int(rng.integers(8, 90)))
+            new.append((sc, cam2, l, p, w2, h2, C.alloc_outputs(w2, h2)))
+        ref = _reference(new)
+        batch.update(C.make_jobs(new))
+        batch.launch()
+        torch.cuda.synchronize()
+        _check(new, ref)
+
+
+def test_autotune_keeps_results(gpu):
+    light, ph = C.light(S.LIGHT_TOWARD), C.phong(*S.PHONG)
+    mix = S.batch_mix(20, seed=17, resolutions=(64, 128))
+    specs = _specs_for(mix, light, ph)
+    ref = _reference(specs)
+    batch = C.Batch(C.make_jobs(specs))
+    t = batch.autotune(reps=1)
+    print(f"\nchosen block={t.block} walk={t.tile_walk} heaviest_first={t.heaviest_first} ms={t.ms:.3f}")
+    assert t.block in (256, 512, 1024) and t.tile_walk in (0, 1) and t.ms > 0
+    for spec in specs:
+        for x in spec[6].values():
+            x.fill_(0)
+    batch.launch()
+    torch.cuda.synchronize()
+    _check(specs, ref)
