@@ -67,7 +67,7 @@ def _time_ms(torch, fn, reps):
     return a.elapsed_time(b) / reps
 
 
-def bench_batched(torch, C, S, reps):
+def bench_batched(torch, C, S, reps, coop):
     """The 256-tree mix: per-scene launches vs one batched launch vs the
     captured (hipGraph) batch after autotuning."""
     mix = S.batch_mix(256, seed=256)
@@ -80,12 +80,12 @@ def bench_batched(torch, C, S, reps):
 
     def plain():
         for sc, cam, l, p, w, h, o in specs:
-            sc.render(cam, l, p, w, h, outputs=o)
+            sc.render(cam, l, p, w, h, outputs=o, **coop)
 
     jobs = C.make_jobs(specs)
     plain_ms = _time_ms(torch, plain, reps)
-    batch_ms = _time_ms(torch, lambda: C.render_batch(jobs), reps)
-    batch = C.Batch(jobs)
+    batch_ms = _time_ms(torch, lambda: C.render_batch(jobs, **coop), reps)
+    batch = C.Batch(jobs, **coop)
     tune = batch.autotune(reps=2)
     graph_ms = _time_ms(torch, batch.launch, reps)
     batch.close()
@@ -97,7 +97,7 @@ def bench_batched(torch, C, S, reps):
             "ms": {"plain": round(plain_ms, 2), "batched": round(batch_ms, 2), "graph": round(graph_ms, 2)}}
 
 
-def bench_serving(C, S, lambdas, seconds):
+def bench_serving(C, S, lambdas, seconds, coop):
     """Poisson arrivals of (tree, orbit camera, size) requests against 64
     resident trees; latency = submit -> result ready."""
     import threading
@@ -107,7 +107,7 @@ def bench_serving(C, S, lambdas, seconds):
     sizes = [(256, 256), (512, 512), (1024, 1024), (1920, 1080)]
     out = []
     for lam in lambdas:
-        server = C.Server(budget_bytes=16 << 30, streams=3, max_batch_jobs=256)
+        server = C.Server(budget_bytes=16 << 30, streams=3, max_batch_jobs=256, **coop)
         mix = S.batch_mix(64, seed=64)
         scenes = [C.Scene(t) for t, _, _ in mix]
         ids = [server.add_scene(s) for s in scenes]
@@ -167,7 +167,16 @@ def main():
     ap.add_argument("--cpu-row-step", type=int, default=1)
     ap.add_argument("--no-extra", action="store_true", help="skip the batched and serving legs")
     ap.add_argument("--serve-seconds", type=float, default=3.0)
+    ap.add_argument("--coop-budget", type=int, default=0,
+                    help="steps before a ray may go to the cooperative kernel (0 default, <0 never)")
+    ap.add_argument("--coop-when", type=int, choices=(0, 1), default=0,
+                    help="0: hand rays over only once every tile is taken; 1: as soon as the budget is spent")
+    ap.add_argument("--coop-grace", type=int, default=0,
+                    help="tail hand-over waits this percent of the tile hand-out time (0 default, <0 none)")
+    ap.add_argument("--only", choices=("plain", "batched", "serving"), default=None,
+                    help="run one leg (plain still sets the headline value)")
     args = ap.parse_args()
+    coop = {"coop": args.coop_budget, "coop_when": args.coop_when, "coop_grace": args.coop_grace}
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
         sys.exit(2)
@@ -185,7 +194,7 @@ def main():
     ph = C.phong(*S.PHONG)
     mode = C.MODE_FAST if args.mode == "fast" else C.MODE_EXACT
     scene = C.Scene(text)
-    plan = scene.plan(mode=mode)
+    plan = scene.plan(mode=mode, coop=args.coop_budget)
     outs = {
         "rgba": torch.empty((h, w, 4), dtype=torch.float32, device=dev),
         "depth": torch.empty((h, w), dtype=torch.float32, device=dev),
@@ -194,7 +203,7 @@ def main():
     }
     stream = torch.cuda.current_stream(dev)
     for _ in range(args.warmup):
-        scene.render(cam, light, ph, w, h, mode=mode, outputs=outs)
+        scene.render(cam, light, ph, w, h, mode=mode, outputs=outs, **coop)
     torch.cuda.synchronize()
 
     starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
@@ -203,7 +212,7 @@ def main():
     t0 = time.perf_counter()
     for i in range(args.steps):
         starts[i].record(stream)
-        scene.render(cam, light, ph, w, h, mode=mode, outputs=outs)
+        scene.render(cam, light, ph, w, h, mode=mode, outputs=outs, **coop)
         ends[i].record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
@@ -232,8 +241,10 @@ def main():
 
     extra = {}
     if not args.no_extra:
-        extra["batched"] = bench_batched(torch, C, S, reps=3)
-        extra["serving"] = bench_serving(C, S, (200, 1000, 5000), args.serve_seconds)
+        if args.only in (None, "batched"):
+            extra["batched"] = bench_batched(torch, C, S, 3, coop)
+        if args.only in (None, "serving"):
+            extra["serving"] = bench_serving(C, S, (200, 1000, 5000), args.serve_seconds, coop)
 
     cpu = None
     if not args.no_cpu:
@@ -248,7 +259,9 @@ def main():
         "config": {"workload": "plain: cheese 512 spheres, 1920x1080, 1 ray/pixel", "mode": args.mode,
                    "rays_per_step": rays, "p50_ms": round(p50, 4), "p99_ms": round(p99, 4),
                    "wall_s": round(wall, 3), "block": plan.block, "grid": plan.grid,
-                   "scene_in_lds": bool(plan.scene_in_lds), "lds_bytes": plan.lds_bytes},
+                   "scene_in_lds": bool(plan.scene_in_lds), "lds_bytes": plan.lds_bytes,
+                   "coop_budget": plan.coop_budget, "coop_when": args.coop_when, "coop_grace": args.coop_grace,
+                   "coop_grid": plan.coop_grid, "coop_block": plan.coop_block},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -55,7 +55,8 @@ class SceneInfo(ctypes.Structure):
 class RenderOpts(ctypes.Structure):
     _fields_ = [("mode", ctypes.c_int32), ("block", ctypes.c_int32), ("scene_placement", ctypes.c_int32),
                 ("counters", ctypes.c_int32), ("counter_out", ctypes.c_void_p), ("tile_order", ctypes.c_void_p),
-                ("samples", ctypes.c_int32), ("seed", ctypes.c_uint32), ("tile_walk", ctypes.c_int32)]
+                ("samples", ctypes.c_int32), ("seed", ctypes.c_uint32), ("tile_walk", ctypes.c_int32),
+                ("coop_budget", ctypes.c_int32), ("coop_when", ctypes.c_int32), ("coop_grace", ctypes.c_int32)]
 
 
 class Job(ctypes.Structure):
@@ -65,7 +66,9 @@ class Job(ctypes.Structure):
 
 class LaunchInfo(ctypes.Structure):
     _fields_ = [("block", ctypes.c_int32), ("grid", ctypes.c_int32), ("lds_bytes", ctypes.c_int64),
-                ("scene_in_lds", ctypes.c_int32), ("compute_units", ctypes.c_int32), ("lds_limit", ctypes.c_int64)]
+                ("scene_in_lds", ctypes.c_int32), ("compute_units", ctypes.c_int32), ("lds_limit", ctypes.c_int64),
+                ("coop_budget", ctypes.c_int32), ("coop_grid", ctypes.c_int32), ("coop_block", ctypes.c_int32),
+                ("reserved0", ctypes.c_int32), ("coop_lds_bytes", ctypes.c_int64)]
 
 
 class BatchTuning(ctypes.Structure):
@@ -75,7 +78,8 @@ class BatchTuning(ctypes.Structure):
 
 class ServerConfig(ctypes.Structure):
     _fields_ = [("budget_bytes", ctypes.c_int64), ("streams", ctypes.c_int32), ("max_batch_jobs", ctypes.c_int32),
-                ("mode", ctypes.c_int32)]
+                ("mode", ctypes.c_int32), ("coop_budget", ctypes.c_int32), ("coop_when", ctypes.c_int32),
+                ("coop_grace", ctypes.c_int32)]
 
 
 class Request(ctypes.Structure):
@@ -125,6 +129,7 @@ _SIGNATURES = [
                                      ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(Outputs),
                                      ctypes.POINTER(RenderOpts), _P]),
     ("csg_sync", ctypes.c_int, [_P]),
+    ("csg_coop_rays", ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int64)]),
     ("csg_render_plan", ctypes.c_int, [_P, ctypes.POINTER(RenderOpts), ctypes.POINTER(LaunchInfo)]),
     ("csg_jitter", ctypes.c_float, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32]),
     ("csg_render_batch", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(RenderOpts), _P]),
@@ -269,17 +274,22 @@ class Scene:
         _check(lib().csg_scene_export(self.handle, links.ctypes.data, boxes.ctypes.data, prims.ctypes.data))
         return links, boxes, prims
 
-    def plan(self, mode=MODE_FAST, block=0, placement=0) -> LaunchInfo:
+    def plan(self, mode=MODE_FAST, block=0, placement=0, coop=0) -> LaunchInfo:
         info = LaunchInfo()
-        opts = RenderOpts(mode, block, placement, 0, None, None, 0, 0, 0)
+        opts = RenderOpts(mode, block, placement, 0, None, None, 0, 0, 0, coop)
         _check(lib().csg_render_plan(self.handle, ctypes.byref(opts), ctypes.byref(info)))
         return info
 
     def render(self, cam, light_, phong_, width, height, mode=MODE_FAST, outputs=None, counters=False,
-               stream=None, device=None, block=0, placement=0, tile_order=None, samples=1, seed=0, walk=0):
+               stream=None, device=None, block=0, placement=0, tile_order=None, samples=1, seed=0, walk=0, coop=0,
+               coop_when=0, coop_grace=0):
         """Render into torch CUDA tensors; returns dict rgba/depth/prim_id/flags(/counters).
 
-        `outputs` may pass preallocated tensors (same keys) to avoid allocation."""
+        `outputs` may pass preallocated tensors (same keys) to avoid allocation.  `coop` is the
+        step budget after which a ray may move to the cooperative kernel (0 default, < 0 never);
+        `coop_when` 0 moves it only in the frame's tail (once every tile has been taken and a further
+        `coop_grace` percent of the time that took has passed; 0 default, < 0 none), 1 as soon as the
+        budget is spent."""
         import torch
         dev = device or torch.device("cuda", torch.cuda.current_device())
         spp = max(1, int(samples))
@@ -293,7 +303,8 @@ class Scene:
                       for k in ("rgba", "depth", "prim_id", "flags")))
         opts = RenderOpts(mode, block, placement, 1 if counters else 0,
                           outputs["counters"].data_ptr() if counters else None,
-                          tile_order.data_ptr() if tile_order is not None else None, spp, seed, walk)
+                          tile_order.data_ptr() if tile_order is not None else None, spp, seed, walk, coop,
+                          coop_when, coop_grace)
         s = stream if stream is not None else torch.cuda.current_stream(dev).cuda_stream
         _check(lib().csg_render_ex(self.handle, ctypes.byref(cam), ctypes.byref(light_), ctypes.byref(phong_),
                                    int(width), int(height), ctypes.byref(o), ctypes.byref(opts), s))
@@ -321,6 +332,15 @@ def _outputs_struct(outputs):
                      for k in ("rgba", "depth", "prim_id", "flags")))
 
 
+def coop_rays(stream=None):
+    """Rays handed to the cooperative kernel so far on `stream` (waits for it)."""
+    import torch
+    s = stream if stream is not None else torch.cuda.current_stream().cuda_stream
+    v = ctypes.c_int64()
+    _check(lib().csg_coop_rays(s, ctypes.byref(v)))
+    return v.value
+
+
 def jitter(seed, pixel, sample, axis):
     return float(lib().csg_jitter(seed, pixel, sample, axis))
 
@@ -339,10 +359,11 @@ def batch_order(jobs):
     return list(order)
 
 
-def render_batch(jobs, mode=MODE_FAST, block=0, walk=0, samples=1, seed=0, stream=None, counters_out=None):
+def render_batch(jobs, mode=MODE_FAST, block=0, walk=0, samples=1, seed=0, stream=None, coop=0, coop_when=0,
+                 coop_grace=0):
     """One persistent launch over a ctypes Job array (see make_jobs)."""
     import torch
-    opts = RenderOpts(mode, block, PLACE_GLOBAL, 0, None, None, samples, seed, walk)
+    opts = RenderOpts(mode, block, PLACE_GLOBAL, 0, None, None, samples, seed, walk, coop, coop_when, coop_grace)
     s = stream if stream is not None else torch.cuda.current_stream().cuda_stream
     _check(lib().csg_render_batch(jobs, len(jobs), ctypes.byref(opts), s))
 
@@ -350,8 +371,8 @@ def render_batch(jobs, mode=MODE_FAST, block=0, walk=0, samples=1, seed=0, strea
 class Batch:
     """A captured batch (csg_batch*): jobs fixed at creation, graph replayed by launch()."""
 
-    def __init__(self, jobs, mode=MODE_FAST, block=0, walk=0, samples=1, seed=0):
-        opts = RenderOpts(mode, block, PLACE_GLOBAL, 0, None, None, samples, seed, walk)
+    def __init__(self, jobs, mode=MODE_FAST, block=0, walk=0, samples=1, seed=0, coop=0, coop_when=0, coop_grace=0):
+        opts = RenderOpts(mode, block, PLACE_GLOBAL, 0, None, None, samples, seed, walk, coop, coop_when, coop_grace)
         h = _P()
         _check(lib().csg_batch_create(jobs, len(jobs), ctypes.byref(opts), ctypes.byref(h)))
         self.handle = h
@@ -388,9 +409,11 @@ class Batch:
 class Server:
     """csg_server*: resident scenes, request queue, continuous batching."""
 
-    def __init__(self, budget_bytes=0, streams=0, max_batch_jobs=0, mode=MODE_FAST):
+    def __init__(self, budget_bytes=0, streams=0, max_batch_jobs=0, mode=MODE_FAST, coop=0, coop_when=0,
+                 coop_grace=0):
         h = _P()
-        cfg = ServerConfig(int(budget_bytes), int(streams), int(max_batch_jobs), int(mode))
+        cfg = ServerConfig(int(budget_bytes), int(streams), int(max_batch_jobs), int(mode), int(coop), int(coop_when),
+                           int(coop_grace))
         _check(lib().csg_server_create(ctypes.byref(cfg), ctypes.byref(h)))
         self.handle = h
         self._scenes = []

@@ -25,9 +25,11 @@ struct Chunk {
   csgray::Job* hostJobs = nullptr;  // pinned
   int32_t cap = 0;                  // jobs the table holds
   uint32_t* queue = nullptr;
+  uint2* heavy = nullptr;
   csgray::RenderParams params{};
   csgray::LaunchShape shape{};
   hipGraphNode_t node = nullptr;
+  hipGraphNode_t coopNode = nullptr;
 };
 
 }  // namespace
@@ -51,6 +53,7 @@ csg_status AllocChunk(Chunk& c, int32_t cap) {
   if (e == hipSuccess) e = hipHostMalloc(&c.hostJobs, sizeof(csgray::Job) * cap, hipHostMallocDefault);
   if (e == hipSuccess) e = hipMalloc(&c.queue, 256);
   if (e == hipSuccess) e = hipMemset(c.queue, 0, 256);
+  if (e == hipSuccess) e = hipMalloc(&c.heavy, sizeof(uint2) * csgray::kHeavyCap);
   if (e != hipSuccess) return e == hipErrorOutOfMemory ? Fail(CSG_E_OOM, "batch allocation failed") : FailHip(e, "batch allocation");
   c.cap = cap;
   return CSG_OK;
@@ -61,6 +64,7 @@ void FreeChunks(csg_batch* b) {
     if (c.devJobs) (void)hipFree(c.devJobs);
     if (c.hostJobs) (void)hipHostFree(c.hostJobs);
     if (c.queue) (void)hipFree(c.queue);
+    if (c.heavy) (void)hipFree(c.heavy);
   }
   b->chunks.clear();
 }
@@ -91,9 +95,10 @@ csg_status Stage(csg_batch* b) {
     const int32_t cnt = std::max(0, std::min(n - start, csgray::kMaxBatchJobs));
     if (cnt > ch.cap) return Fail(CSG_E_ARG, "csg_batch: chunk capacity exceeded");
     int64_t base = 0;
-    int32_t depth = 1;
+    int32_t depth = 1, maxOps = 1;
     for (int32_t i = 0; i < cnt; i++) {
       const csg_job& j = b->jobs[order[start + i]];
+      maxOps = std::max(maxOps, j.scene->dev.nOps);
       const csg_status st = BuildJob(j.scene, &j.camera, &j.light, &j.phong, j.width, j.height, &j.out, &b->opts,
                                      b->device, base, &ch.hostJobs[i]);
       if (st != CSG_OK) return st;
@@ -112,6 +117,7 @@ csg_status Stage(csg_batch* b) {
     const hipError_t e = csgray::ChooseShape(depth, 0, b->cfg, &ch.shape);
     if (e != hipSuccess) return FailHip(e, "choosing the launch shape");
     p.totalWaves = static_cast<uint32_t>(ch.shape.grid) * static_cast<uint32_t>(ch.shape.block / 64);
+    PrepareCoop(b->cfg, maxOps, 0, ch.heavy, &p);
   }
   return CSG_OK;
 }
@@ -168,24 +174,42 @@ csg_status Capture(csg_batch* b) {
   std::vector<hipGraphNode_t> nodes(count);
   e = hipGraphGetNodes(b->graph, nodes.data(), &count);
   if (e != hipSuccess) return FailHip(e, "hipGraphGetNodes");
-  for (Chunk& ch : b->chunks) ch.node = nullptr;
+  for (Chunk& ch : b->chunks) {
+    ch.node = nullptr;
+    ch.coopNode = nullptr;
+  }
   for (hipGraphNode_t nd : nodes) {
     hipGraphNodeType type;
     if (hipGraphNodeGetType(nd, &type) != hipSuccess || type != hipGraphNodeTypeKernel) continue;
     hipKernelNodeParams kp{};
     if (hipGraphKernelNodeGetParams(nd, &kp) != hipSuccess || !kp.kernelParams) continue;
     const auto* p = static_cast<const csgray::RenderParams*>(kp.kernelParams[0]);
-    for (Chunk& ch : b->chunks)
-      if (p->queue == ch.queue) ch.node = nd;
+    for (Chunk& ch : b->chunks) {
+      if (p->queue != ch.queue) continue;
+      // the render kernel runs on the chunk's launch shape, the cooperative one on 64-lane groups
+      if (kp.blockDim.x == 64 && ch.shape.block != 64) ch.coopNode = nd;
+      else ch.node = nd;
+    }
   }
-  for (const Chunk& ch : b->chunks)
+  for (const Chunk& ch : b->chunks) {
     if (!ch.node) return Fail(CSG_E_HIP, "captured graph lacks a render node");
+    if ((ch.params.budget > 0) != (ch.coopNode != nullptr)) return Fail(CSG_E_HIP, "captured graph lacks a node");
+  }
   return CSG_OK;
 }
 
 // Points each kernel node of the instantiated graph at its chunk's current
-// launch parameters.
+// launch parameters.  The graph is captured again instead when the nodes no
+// longer match the chunks (another chunk count, deferral switched on or off)
+// or a kernel needs more than 64 KB of LDS, whose launch attribute only a
+// direct launch sets.
 csg_status PatchNodes(csg_batch* b) {
+  for (const Chunk& ch : b->chunks) {
+    const size_t coopLds = csgray::CoopLaunchLds(ch.params);
+    if (!ch.node || (ch.params.budget > 0) != (ch.coopNode != nullptr) || ch.shape.lds > 64 * 1024 ||
+        (ch.params.budget > 0 && coopLds > 64 * 1024))
+      return Capture(b);
+  }
   for (Chunk& ch : b->chunks) {
     hipKernelNodeParams kp{};
     hipError_t e = hipGraphKernelNodeGetParams(ch.node, &kp);
@@ -198,6 +222,19 @@ csg_status PatchNodes(csg_batch* b) {
     kp.sharedMemBytes = static_cast<unsigned int>(ch.shape.lds);
     e = hipGraphExecKernelNodeSetParams(b->exec, ch.node, &kp);
     if (e != hipSuccess) return FailHip(e, "hipGraphExecKernelNodeSetParams");
+    if (ch.coopNode) {
+      hipKernelNodeParams cp{};
+      e = hipGraphKernelNodeGetParams(ch.coopNode, &cp);
+      if (e != hipSuccess) return FailHip(e, "hipGraphKernelNodeGetParams");
+      void* cargs[1] = {&ch.params};
+      cp.kernelParams = cargs;
+      cp.extra = nullptr;
+      cp.gridDim = dim3(ch.params.coopWaves / static_cast<uint32_t>(ch.params.coopBlock / 64));
+      cp.blockDim = dim3(ch.params.coopBlock);
+      cp.sharedMemBytes = static_cast<unsigned int>(csgray::CoopLaunchLds(ch.params));
+      e = hipGraphExecKernelNodeSetParams(b->exec, ch.coopNode, &cp);
+      if (e != hipSuccess) return FailHip(e, "hipGraphExecKernelNodeSetParams");
+    }
   }
   return CSG_OK;
 }

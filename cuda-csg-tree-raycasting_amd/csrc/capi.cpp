@@ -99,6 +99,82 @@ csg_status QueueFor(int dev, hipStream_t stream, uint32_t** out) {
   return CSG_OK;
 }
 
+std::mutex g_heavyMutex;
+std::map<std::pair<int, hipStream_t>, uint2*> g_heavy;
+
+csg_status HeavyFor(int dev, hipStream_t stream, uint2** out) {
+  std::lock_guard<std::mutex> lock(g_heavyMutex);
+  auto it = g_heavy.find({dev, stream});
+  if (it != g_heavy.end()) {
+    *out = it->second;
+    return CSG_OK;
+  }
+  void* h = nullptr;
+  const hipError_t e = hipMalloc(&h, sizeof(uint2) * csgray::kHeavyCap);
+  if (e != hipSuccess) return e == hipErrorOutOfMemory ? Fail(CSG_E_OOM, "heavy-ray list allocation failed")
+                                                       : FailHip(e, "hipMalloc");
+  g_heavy[{dev, stream}] = static_cast<uint2*>(h);
+  *out = static_cast<uint2*>(h);
+  return CSG_OK;
+}
+
+// Budget (0: no deferral), grid and LDS of the cooperative kernel for scenes
+// of at most `maxOps` operators.  `sceneLds` > 0 (plain launches whose render
+// stages the scene in LDS) lets the cooperative workgroups stage it too, with
+// as many one-ray waves per workgroup as the rest of the LDS holds.
+struct CoopPlan {
+  int32_t budget = 0;
+  uint32_t waves = 0;
+  uint32_t queueCap = 0;
+  int32_t block = 64;
+  bool lds = false;
+  size_t ldsBytes = 0;
+};
+
+CoopPlan PlanCoop(const LaunchConfig& cfg, int32_t maxOps, size_t sceneLds) {
+  CoopPlan c;
+  if (cfg.budget < 0 || maxOps <= 0) return c;
+  const size_t wave = CoopLdsBytes(maxOps, &c.queueCap);
+  if (wave == 0) return c;
+  int cus = 0;
+  size_t ldsPerBlock = 0;
+  if (DeviceShapeLimits(&cus, &ldsPerBlock) != hipSuccess || cus <= 0) return c;
+  const size_t limit = (ldsPerBlock ? ldsPerBlock : 65536) - 1024;  // static LDS and slack
+  const size_t ldsPerCu = 160 * 1024;
+  c.budget = cfg.budget > 0 ? cfg.budget : kDefaultBudget;
+  if (sceneLds > 0 && sceneLds + wave <= limit) {
+    // one workgroup per CU holding the scene and up to 16 waves
+    const size_t w = std::min<size_t>(16, (limit - sceneLds) / wave);
+    c.lds = true;
+    c.block = static_cast<int32_t>(64 * w);
+    c.ldsBytes = sceneLds + w * wave;
+    c.waves = static_cast<uint32_t>(static_cast<size_t>(cus) * w);
+  } else {
+    // one-wave workgroups over the scene in global memory, up to 16 per CU
+    const size_t perCu = std::max<size_t>(1, std::min<size_t>(16, ldsPerCu / (wave + 256)));
+    c.waves = static_cast<uint32_t>(static_cast<size_t>(cus) * perCu);
+    c.ldsBytes = wave;
+  }
+  return c;
+}
+
+void PrepareCoop(const LaunchConfig& cfg, int32_t maxOps, size_t sceneLds, uint2* heavy, RenderParams* p) {
+  p->budget = 0;
+  if (!heavy) return;
+  const CoopPlan c = PlanCoop(cfg, maxOps, sceneLds);
+  if (c.budget <= 0) return;
+  p->budget = c.budget;
+  p->heavy = heavy;
+  p->heavyCap = kHeavyCap;
+  p->coopOps = maxOps;
+  p->coopQueueCap = c.queueCap;
+  p->coopWaves = c.waves;
+  p->coopBlock = c.block;
+  p->coopLds = c.lds ? 1 : 0;
+  p->coopEager = cfg.coopEager ? 1 : 0;
+  p->coopGrace = cfg.coopGrace > 0 ? cfg.coopGrace : (cfg.coopGrace < 0 ? 0 : kDefaultCoopGrace);
+}
+
 }  // namespace rt
 }  // namespace csgray
 
@@ -306,6 +382,11 @@ csg_status ParseOpts(const csg_render_opts* opts, csgray::LaunchConfig* cfg) {
   cfg->block = opts->block;
   cfg->sceneInLds = opts->scene_placement == 0 ? -1 : (opts->scene_placement == 2 ? 1 : 0);
   cfg->morton = opts->tile_walk == 1;
+  cfg->budget = opts->coop_budget;
+  if (opts->coop_when != 0 && opts->coop_when != 1) return Fail(CSG_E_ARG, "coop_when must be 0 or 1");
+  cfg->coopEager = opts->coop_when == 1;
+  if (opts->coop_grace > 100000) return Fail(CSG_E_ARG, "coop_grace above 100000 percent");
+  cfg->coopGrace = opts->coop_grace;
   return CSG_OK;
 }
 
@@ -437,6 +518,29 @@ csg_status csg_render_plan(const csg_scene* s, const csg_render_opts* opts, csg_
   info->scene_in_lds = shape.sceneInLds ? 1 : 0;
   info->compute_units = cus;
   info->lds_limit = static_cast<int64_t>(ldsLimit);
+  const CoopPlan c = PlanCoop(cfg, s->dev.nOps, shape.sceneInLds ? SceneBytes(s) : 0);
+  info->coop_budget = c.budget;
+  info->coop_grid = static_cast<int32_t>(c.waves / static_cast<uint32_t>(c.block / 64));
+  info->coop_block = c.block;
+  info->coop_lds_bytes = static_cast<int64_t>(c.ldsBytes);
+  return CSG_OK;
+}
+
+csg_status csg_coop_rays(void* stream, int64_t* rays) {
+  if (!rays) return Fail(CSG_E_ARG, "csg_coop_rays: NULL argument");
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return FailHip(e, "hipGetDevice");
+  const hipStream_t hs = static_cast<hipStream_t>(stream);
+  uint32_t* q = nullptr;
+  const csg_status qs = QueueFor(dev, hs, &q);
+  if (qs != CSG_OK) return qs;
+  e = hipStreamSynchronize(hs);
+  if (e != hipSuccess) return FailHip(e, "hipStreamSynchronize");
+  unsigned long long v = 0;
+  e = hipMemcpy(&v, q + 6, sizeof v, hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return FailHip(e, "hipMemcpy");
+  *rays = static_cast<int64_t>(v);
   return CSG_OK;
 }
 
@@ -470,6 +574,12 @@ csg_status csg_render_ex(const csg_scene* s, const csg_camera* cam, const csg_li
   const hipStream_t hs = static_cast<hipStream_t>(stream);
   const csg_status qs = QueueFor(dev, hs, &p.queue);
   if (qs != CSG_OK) return qs;
+  uint2* heavy = nullptr;
+  if (cfg.budget >= 0) {
+    const csg_status hsd = HeavyFor(dev, hs, &heavy);
+    if (hsd != CSG_OK) return hsd;
+  }
+  PrepareCoop(cfg, s->dev.nOps, shape.sceneInLds ? SceneBytes(s) : 0, heavy, &p);
   e = csgray::LaunchRender(p, cfg, shape, hs);
   if (e != hipSuccess) return FailHip(e, "render launch");
   return CSG_OK;
@@ -532,6 +642,14 @@ csg_status csg_render_batch(const csg_job* jobs, int32_t njobs, const csg_render
     p.totalWaves = static_cast<uint32_t>(shape.grid) * static_cast<uint32_t>(shape.block / 64);
     const csg_status qs = QueueFor(dev, hs, &p.queue);
     if (qs != CSG_OK) return qs;
+    uint2* heavy = nullptr;
+    if (cfg.budget >= 0) {
+      const csg_status hsd = HeavyFor(dev, hs, &heavy);
+      if (hsd != CSG_OK) return hsd;
+    }
+    int32_t maxOps = 1;
+    for (int32_t i = 0; i < n; i++) maxOps = std::max(maxOps, jobs[start + i].scene->dev.nOps);
+    PrepareCoop(cfg, maxOps, 0, heavy, &p);
     e = csgray::LaunchRender(p, cfg, shape, hs);
     if (e != hipSuccess) return FailHip(e, "batched render launch");
   }

@@ -56,8 +56,24 @@ struct RenderParams {
   uint32_t* queue;            // [0] next tile, [1] waves finished; zero between launches
   const uint32_t* tileOrder;  // optional: order in which queue entries map to tiles (plain)
   int32_t morton;             // 1: tiles of a job are visited in Morton order
-  int32_t pad_;
+  // Cooperative evaluation of heavy rays: a ray still running after `budget`
+  // steps once every tile has been taken (at once when `coopEager`) is
+  // appended to `heavy` ({job, output index}); a second kernel evaluates each
+  // one with a whole wave.  budget <= 0: never defer.
+  int32_t budget;
+  int32_t coopEager;
+  int32_t coopGrace;          // tail mode: percent of the tile hand-out time to wait after the last tile
+  uint2* heavy;
+  uint32_t heavyCap;
+  int32_t coopOps;            // operator-state slots per wave (max operators of the launch)
+  uint32_t coopQueueCap;      // task ring entries (power of two >= coopOps)
+  uint32_t coopWaves;         // waves in the cooperative grid
+  int32_t coopBlock;          // threads per cooperative workgroup (a multiple of 64)
+  int32_t coopLds;            // 1: the cooperative workgroups stage the scene in LDS (plain launches)
 };
+// queue words: [0] next tile, [1] tile waves done, [2] heavy rays, [3] next heavy ray, [4] coop waves done,
+// [6..7] heavy rays handed over on this stream so far (64-bit); on their own cache line, [32..33] time
+// the first tile was taken, [34..35] time the tiles ran out (64-bit wall clock | 1, 0 = not yet)
 
 // Jitter of sample s of pixel p: a 32-bit integer hash of (seed, p, s)
 // mapped to [0, 1) with 24 bits; identical on host and device.
@@ -80,6 +96,9 @@ struct LaunchConfig {
   int32_t block = 0;   // 0 = choose
   int32_t sceneInLds = -1;  // -1 = choose, 0 = global, 1 = LDS
   bool morton = false;      // visit each job's tiles in Morton order
+  int32_t budget = 0;       // 0: default step budget, < 0: no deferral, > 0: steps before deferral
+  bool coopEager = false;   // hand rays over as soon as the budget is spent, not only after the queue drains
+  int32_t coopGrace = 0;    // 0: default; < 0: none; else percent (see RenderParams::coopGrace)
 };
 
 // Launch geometry chosen for a scene: workgroup size, grid, LDS bytes.
@@ -105,6 +124,19 @@ hipError_t LaunchRender(const RenderParams& p, const LaunchConfig& cfg, const La
 
 // Most jobs one batched launch takes (their tile prefix is staged in LDS).
 constexpr int32_t kMaxBatchJobs = 1024;
+
+// Default per-lane step budget before a ray is handed to the cooperative
+// kernel, and the capacity of the heavy-ray list.
+constexpr int32_t kDefaultBudget = 384;
+constexpr int32_t kDefaultCoopGrace = 50;
+constexpr uint32_t kHeavyCap = 1u << 22;
+
+// LDS bytes the cooperative kernel needs for scenes of at most `ops`
+// operators (0 when it cannot run: the caller then disables deferral).
+size_t CoopLdsBytes(int32_t ops, uint32_t* queueCap);
+// LDS of one wave's cooperative state, and of one cooperative workgroup of `p`.
+size_t CoopWaveLds(int32_t ops, uint32_t queueCap);
+size_t CoopLaunchLds(const RenderParams& p);
 
 // Largest operator depth the traversal stack supports.
 int32_t MaxSupportedDepth();

@@ -318,9 +318,22 @@ __device__ __forceinline__ Hit Intersect(const SceneView& sv, const RayF& r, uin
 //   * decisions are selects and non-short-circuit boolean logic; stack
 //     indices are clamped to the slab and an out-of-range access marks the
 //     ray aborted instead of branching.
+//
+// Hand-over: once a ray has run `budget` micro-steps the trace may stop with
+// `exceeded` set, and the caller passes the ray to the cooperative kernel.
+// With `gstamps` null that happens at once.  Otherwise `gstamps` holds the
+// launch's start time and the time its last tile was handed out (0 until
+// then), and `lstamps` the workgroup's LDS copy of them (set by its first
+// wave that finds no tile left); the ray goes only when the time since the
+// last tile has reached `grace` percent of the time between the two: the
+// frame's tail.  Lanes of a wave enter together and step together, so
+// `iter` is the same in every active lane and the check (at `budget` steps,
+// then every 128; the global stamps every 256) is wave-uniform.
 template <bool kFast, bool kCount>
 __device__ Hit Trace(const SceneView& sv, const RayF& r, uint32_t rootDesc, uint2* stk, int stride, int cap,
-                     const uint16_t* table, Counts& n) {
+                     const uint16_t* table, Counts& n, int budget, const uint64_t* lstamps,
+                     const uint64_t* gstamps, int grace, bool& exceeded) {
+  exceeded = false;
   if (DescType(rootDesc) >= kSphereT) {
     // A primitive root: GoTo intersects it with tmin 0 and the state machine
     // ends without changing the record's classification.
@@ -367,7 +380,7 @@ __device__ Hit Trace(const SceneView& sv, const RayF& r, uint32_t rootDesc, uint
         const float4 q3 = sv.ops[4 * k + 3];
         lDesc = AsU(q3.x);
         rDesc = AsU(q3.y);
-        opType = AsU(q3.z);
+        opType = AsU(q3.z) & 0xFFu;
         opParent = AsU(q3.w);
         const bool gL = BoxHit<kFast>(r, q0.x, q0.y, q0.z, q1.x, q1.y, q1.z, tmin);
         const bool gR = BoxHit<kFast>(r, q0.w, q1.w, q2.x, q2.y, q2.z, q2.w, tmin);
@@ -439,7 +452,7 @@ __device__ Hit Trace(const SceneView& sv, const RayF& r, uint32_t rootDesc, uint
         const uint32_t code = e.y & kSlotMask;
         lDesc = AsU(pq.x);
         rDesc = AsU(pq.y);
-        opType = AsU(pq.z);
+        opType = AsU(pq.z) & 0xFFu;
         opParent = AsU(pq.w);
         if (code == kSlotSaveLft) {
           // SaveLft: tmin parked by the GoTo that went left comes back and
@@ -454,9 +467,27 @@ __device__ Hit Trace(const SceneView& sv, const RayF& r, uint32_t rootDesc, uint
       }
     }
     iter++;
-    done = done | aborted | (iter >= kIterCap);
+    bool late = false;
+    if (iter >= budget && (iter == budget || (iter & 127) == 0)) {
+      if (!gstamps) {
+        late = true;
+      } else {
+        // the workgroup's copy (LDS), or every 1024 steps the launch's own
+        uint64_t startAt = lstamps[0], drainAt = lstamps[1];
+        if (drainAt == 0 && (iter & 255) == 0) {
+          startAt = __hip_atomic_load(gstamps, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          drainAt = __hip_atomic_load(gstamps + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        if (drainAt != 0) {
+          const uint64_t now = wall_clock64();
+          late = (now - drainAt) * 100u >= (drainAt - startAt) * static_cast<uint64_t>(grace);
+        }
+      }
+    }
+    exceeded = !done & !aborted & late;
+    done = done | aborted | exceeded | (iter >= kIterCap);
   }
-  if (aborted || iter >= kIterCap) L = Hit{-1.0f, kMissBit | kAborted | 0xFFFF0000u};
+  if (aborted || (!exceeded && iter >= kIterCap)) L = Hit{-1.0f, kMissBit | kAborted | 0xFFFF0000u};
   return L;
 }
 
@@ -584,7 +615,8 @@ __device__ __forceinline__ uint32_t Compact1By1(uint32_t v) {
 // Traces and shades the 64 rays of one tile (one per lane).
 template <bool kFast, bool kCount>
 __device__ __forceinline__ void RenderTile(const Job& jb, const SceneView& sv, uint32_t local, bool morton,
-                                           uint2* slab, int stride, int cap, const uint16_t* table, int lane) {
+                                           uint2* slab, int stride, int cap, const uint16_t* table, int lane,
+                                           const RenderParams& p, uint32_t jobIndex, const uint64_t* lstamps) {
   const uint32_t perSample = static_cast<uint32_t>(jb.tiles / jb.spp);
   const uint32_t sample = local / perSample;
   uint32_t t = local - sample * perSample;
@@ -608,8 +640,23 @@ __device__ __forceinline__ void RenderTile(const Job& jb, const SceneView& sv, u
   }
   const RayF r = MakeRay(jb, x, y, ox, oy);
   Counts n{0, 0, 0, 0, 0, 0};
-  const Hit h = Trace<kFast, kCount>(sv, r, jb.sc.rootDesc, slab, stride, cap, table, n);
   const size_t out = static_cast<size_t>(pixel) * static_cast<size_t>(jb.spp) + sample;
+  bool exceeded = false;
+  const int budget = p.budget > 0 ? p.budget : 0x7FFFFFFF;
+  const uint64_t* gstamps = p.coopEager ? nullptr : reinterpret_cast<const uint64_t*>(p.queue + 32);
+  Hit h = Trace<kFast, kCount>(sv, r, jb.sc.rootDesc, slab, stride, cap, table, n, budget, lstamps, gstamps,
+                               p.coopGrace, exceeded);
+  if (exceeded) {
+    // hand the ray to the cooperative kernel; if its list is full, finish here
+    const uint32_t slot = atomicAdd(p.queue + 2, 1u);
+    if (slot < p.heavyCap) {
+      p.heavy[slot] = make_uint2(jobIndex, static_cast<uint32_t>(out));
+      return;
+    }
+    n = Counts{0, 0, 0, 0, 0, 0};
+    h = Trace<kFast, kCount>(sv, r, jb.sc.rootDesc, slab, stride, cap, table, n, 0x7FFFFFFF, lstamps, nullptr, 0,
+                             exceeded);
+  }
   ShadeAndStore(jb, sv, r, h, out);
   if (kCount && jb.counters) {
     int32_t* c = jb.counters + 6 * out;
@@ -624,8 +671,10 @@ template <bool kFast, bool kCount, bool kLds, bool kBatch>
 __global__ void __launch_bounds__(1024) RenderKernel(RenderParams p) {
   extern __shared__ uint4 ldsBase[];
   __shared__ uint16_t table[27];
+  __shared__ uint64_t ldsStamps[2];
   const int tid = threadIdx.x;
   const int cap = p.stackSlots > 0 ? p.stackSlots : 1;
+  if (tid < 2) ldsStamps[tid] = 0;
   uint2* slab = reinterpret_cast<uint2*>(ldsBase);
   if (tid < 27) table[tid] = kActionTable[tid];
   int32_t* jobBase = reinterpret_cast<int32_t*>(slab + static_cast<size_t>(cap) * blockDim.x);
@@ -649,11 +698,29 @@ __global__ void __launch_bounds__(1024) RenderKernel(RenderParams p) {
   __syncthreads();
 
   const int lane = tid & 63;
+  uint64_t* stamps = reinterpret_cast<uint64_t*>(p.queue + 32);
   for (;;) {
     uint32_t q = 0;
     if (lane == 0) q = atomicAdd(p.queue, 1u);
     q = __builtin_amdgcn_readfirstlane(q);
-    if (q >= static_cast<uint32_t>(p.numTiles)) break;
+    // time stamps of the first tile and of the first pop past the last one
+    // (every lane stores the same value: no branch on the lane index here)
+    if (q == 0) __hip_atomic_store(stamps, wall_clock64() | 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (q >= static_cast<uint32_t>(p.numTiles)) {
+      uint64_t drainAt = 0;
+      if (q == static_cast<uint32_t>(p.numTiles)) {
+        drainAt = wall_clock64() | 1u;
+        __hip_atomic_store(stamps + 1, drainAt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      } else {
+        drainAt = __hip_atomic_load(stamps + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+      // tell the waves of this workgroup that are still tracing
+      if (drainAt != 0 && ldsStamps[1] == 0) {
+        ldsStamps[0] = __hip_atomic_load(stamps, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        ldsStamps[1] = drainAt;
+      }
+      break;
+    }
     if (kBatch) {
       // the job whose queue range holds q: last tileBase <= q
       int lo = 0, hi = p.numJobs - 1;
@@ -666,23 +733,282 @@ __global__ void __launch_bounds__(1024) RenderKernel(RenderParams p) {
       const Job& jb = p.jobs[j];
       const SceneView jsv{jb.sc.ops, jb.sc.primA, jb.sc.primB};
       RenderTile<kFast, kCount>(jb, jsv, q - static_cast<uint32_t>(jb.tileBase), p.morton != 0, slab + tid,
-                                blockDim.x, cap, table, lane);
+                                blockDim.x, cap, table, lane, p, static_cast<uint32_t>(j), ldsStamps);
     } else {
       const uint32_t t = p.tileOrder ? p.tileOrder[q] : q;
-      RenderTile<kFast, kCount>(p.job, sv, t, p.morton != 0, slab + tid, blockDim.x, cap, table, lane);
+      RenderTile<kFast, kCount>(p.job, sv, t, p.morton != 0, slab + tid, blockDim.x, cap, table, lane, p, 0u,
+                                ldsStamps);
     }
   }
-  // The last wave out returns the queue to zero for the next launch.
+  // The last wave out returns the queue and the time stamps to zero for the
+  // next launch.
   if (lane == 0) {
     const uint32_t finished = atomicAdd(p.queue + 1, 1u);
     if (finished == p.totalWaves - 1) {
+      __hip_atomic_store(stamps, uint64_t{0}, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(stamps + 1, uint64_t{0}, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       atomicExch(p.queue, 0u);
       atomicExch(p.queue + 1, 0u);
     }
   }
 }
 
+
+// ---------------------------------------------------------------------------
+// Cooperative evaluation of one ray by a whole wave.
+//
+// Evaluating a subtree X with a given tmin is a pure function of (X, tmin):
+// GoTo(X) tests both children's boxes at tmin, intersects primitive children
+// whose box is hit, and evaluates operator children whose box is hit with the
+// same tmin (the reference restores tmin through SaveLft before the right
+// child); Compute(X) then combines the two records, possibly looping into
+// one child with a new tmin (without re-testing that child's box).  A missed
+// child's record is a Miss whose other fields are never read.  So the two
+// children of an operator can be evaluated at the same time, and a wave can
+// run one ray as a dataflow over the tree:
+//   ENTER(X)   = GoTo into X: box tests, primitive tests, ENTER for operator
+//                children whose box is hit, COMPUTE(X) at once if none;
+//   COMPUTE(X) = Compute at X: Ret/Miss delivers the record to the parent
+//                (the second delivery schedules the parent's COMPUTE);
+//                Loop re-enters one child (a primitive inline).
+// Every box test, primitive test and decision of the sequential machine
+// happens exactly once, so results and event counts are identical.  At most
+// one task per operator is alive at a time, so task state is indexed by
+// operator.
+struct OpState {
+  float tmin;
+  uint32_t pending;  // child records still to arrive
+  float lt;
+  uint32_t lb;
+  float rt;
+  uint32_t rb;
+};
+
+// Orders the LDS accesses of the lanes of one wave (waves of a cooperative
+// workgroup run independently, so no workgroup barrier).
+__device__ __forceinline__ void WaveSync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ uint32_t EnterTask(uint32_t op) { return op << 1; }
+__device__ __forceinline__ uint32_t ComputeTask(uint32_t op) { return (op << 1) | 1u; }
+
+template <bool kFast, bool kCount>
+__device__ Hit CoopTrace(const SceneView& sv, const RayF& r, OpState* st, uint32_t* q, uint32_t qmask,
+                         const uint16_t* table, uint32_t* shared, int lane, Counts& n) {
+  if (lane == 0) {
+    st[0].tmin = 0.0f;
+    q[0] = EnterTask(0);
+    shared[0] = 0;
+  }
+  uint32_t head = 0, tail = 1;
+  WaveSync();
+  for (int iter = 0; iter < kIterCap; iter++) {
+    const uint32_t avail = tail - head;
+    if (avail == 0) break;  // nothing alive and no result: a malformed tree
+    const uint32_t task = static_cast<uint32_t>(lane) < avail ? q[(head + lane) & qmask] : kNone;
+    head += avail < 64u ? avail : 64u;
+    uint32_t sp0 = kNone, sp1 = kNone;
+    if (task != kNone) {
+      const uint32_t X = task >> 1;
+      const float4 q3 = sv.ops[4 * X + 3];
+      const uint32_t lDesc = AsU(q3.x), rDesc = AsU(q3.y);
+      if ((task & 1u) == 0) {
+        if (kCount) n.go++;
+        const float tmin = st[X].tmin;
+        const float4 q0 = sv.ops[4 * X + 0];
+        const float4 q1 = sv.ops[4 * X + 1];
+        const float4 q2 = sv.ops[4 * X + 2];
+        const bool gL = BoxHit<kFast>(r, q0.x, q0.y, q0.z, q1.x, q1.y, q1.z, tmin);
+        const bool gR = BoxHit<kFast>(r, q0.w, q1.w, q2.x, q2.y, q2.z, q2.w, tmin);
+        if (kCount) n.box += 2;
+        Hit L = MissRecord(), R = MissRecord();
+        const bool pL = gL & (DescType(lDesc) >= kSphereT);
+        const bool pR = gR & (DescType(rDesc) >= kSphereT);
+        if (pL | pR) {
+#pragma nounroll
+          for (int j = 0; j < 2; j++) {
+            if (j == 0 ? pL : pR) {
+              const Hit h = Intersect<kFast, kCount>(sv, r, j == 0 ? lDesc : rDesc, tmin, j == 0 ? L : R, n);
+              if (j == 0) L = h;
+              else R = h;
+            }
+          }
+        }
+        const bool dL = gL & !pL, dR = gR & !pR;
+        if (dL) {
+          st[DescIndex(lDesc)].tmin = tmin;
+          sp0 = EnterTask(DescIndex(lDesc));
+        } else {
+          st[X].lt = L.t;
+          st[X].lb = L.bits;
+        }
+        if (dR) {
+          st[DescIndex(rDesc)].tmin = tmin;
+          sp1 = EnterTask(DescIndex(rDesc));
+        } else {
+          st[X].rt = R.t;
+          st[X].rb = R.bits;
+        }
+        st[X].pending = static_cast<uint32_t>(dL) + static_cast<uint32_t>(dR);
+        if (!dL & !dR) sp0 = ComputeTask(X);
+      } else {
+        if (kCount) n.comp++;
+        Hit L{st[X].lt, st[X].lb}, R{st[X].rt, st[X].rb};
+        const uint32_t typeWord = AsU(q3.z), parent = AsU(q3.w);
+        const uint32_t acts = table[(typeWord & 0xFFu) * 9u + State(L.bits) * 3u + State(R.bits)];
+        const bool lc = L.t < R.t, rc = L.t > R.t;
+        const bool aRetL = (acts & kRetL) != 0, aRetLc = (acts & kRetLIfCloser) != 0;
+        const bool aRetR = (acts & kRetR) != 0, aRetRc = (acts & kRetRIfCloser) != 0;
+        const bool aLoopL = (acts & kLoopL) != 0, aLoopLc = (acts & kLoopLIfCloser) != 0;
+        const bool aLoopR = (acts & kLoopR) != 0, aLoopRc = (acts & kLoopRIfCloser) != 0;
+        const bool retL = aRetL | (aRetLc & lc);
+        const bool retR = !retL & (aRetR | (aRetRc & rc));
+        const bool loopL = !retL & !retR & (aLoopL | (aLoopLc & lc));
+        const bool loopR = !retL & !retR & !loopL & (aLoopR | (aLoopRc & rc));
+        if (loopL | loopR) {
+          const float tmin = loopL ? L.t : R.t;
+          const uint32_t d = loopL ? lDesc : rDesc;
+          if (DescType(d) >= kSphereT) {
+            // a primitive child: GoTo intersects it, then Compute again
+            if (kCount) n.go++;
+            const Hit h = Intersect<kFast, kCount>(sv, r, d, tmin, loopL ? L : R, n);
+            if (loopL) {
+              st[X].lt = h.t;
+              st[X].lb = h.bits;
+            } else {
+              st[X].rt = h.t;
+              st[X].rb = h.bits;
+            }
+            sp0 = ComputeTask(X);
+          } else {
+            st[DescIndex(d)].tmin = tmin;
+            st[X].pending = 1;
+            sp0 = EnterTask(DescIndex(d));
+          }
+        } else {
+          const uint32_t rb = (acts & kFlipR) ? (R.bits ^ (kFlip | kExit | kEnter)) : R.bits;
+          const Hit res = retL ? L : (retR ? Hit{R.t, rb} : MissRecord());
+          if (parent == kNone) {
+            shared[1] = AsU(res.t);
+            shared[2] = res.bits;
+            shared[0] = 1;
+          } else if (typeWord & 0x100u) {
+            st[parent].rt = res.t;
+            st[parent].rb = res.bits;
+          } else {
+            st[parent].lt = res.t;
+            st[parent].lb = res.bits;
+          }
+          if (parent != kNone && atomicSub(&st[parent].pending, 1u) == 1u) sp0 = ComputeTask(parent);
+        }
+      }
+    }
+    // append the new tasks in lane order
+    const uint64_t b1 = __ballot(sp0 != kNone);
+    const uint64_t b2 = __ballot(sp1 != kNone);
+    const uint32_t below = __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(b1 >> 32),
+                                                     __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(b1), 0u)) +
+                           __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(b2 >> 32),
+                                                     __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(b2), 0u));
+    uint32_t at = tail + below;
+    if (sp0 != kNone) q[(at++) & qmask] = sp0;
+    if (sp1 != kNone) q[at & qmask] = sp1;
+    tail += static_cast<uint32_t>(__popcll(b1) + __popcll(b2));
+    WaveSync();
+    if (shared[0]) break;
+  }
+  const Hit res = shared[0] ? Hit{__uint_as_float(shared[1]), shared[2]} : Hit{-1.0f, kMissBit | kAborted | 0xFFFF0000u};
+  WaveSync();
+  return res;
+}
+
+// Each wave of a workgroup evaluates its own heavy rays.  LDS: when kLds,
+// the scene copy shared by the workgroup (plain launches), then per wave the
+// operator states (coopOps) and the task ring (coopQueueCap).
+template <bool kFast, bool kCount, bool kBatch, bool kLds>
+__global__ void __launch_bounds__(1024) CoopKernel(RenderParams p) {
+  extern __shared__ uint4 ldsBase[];
+  __shared__ uint16_t table[27];
+  __shared__ uint32_t sharedAll[16 * 4];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  if (tid < 27) table[tid] = kActionTable[tid];
+  SceneView sv0{p.job.sc.ops, p.job.sc.primA, p.job.sc.primB};
+  size_t sceneBytes = 0;
+  if (kLds) {
+    float4* lds = reinterpret_cast<float4*>(ldsBase);
+    const int nOps4 = 4 * p.job.sc.nOps, nP = p.job.sc.nPrims;
+    for (int i = tid; i < nOps4; i += blockDim.x) lds[i] = p.job.sc.ops[i];
+    for (int i = tid; i < nP; i += blockDim.x) {
+      lds[nOps4 + i] = p.job.sc.primA[i];
+      lds[nOps4 + nP + i] = p.job.sc.primB[i];
+    }
+    sv0 = SceneView{lds, lds + nOps4, lds + nOps4 + nP};
+    sceneBytes = static_cast<size_t>(nOps4 + 2 * nP) * sizeof(float4);
+  }
+  const size_t perWave = static_cast<size_t>(p.coopOps) * sizeof(OpState) + static_cast<size_t>(p.coopQueueCap) * 4;
+  OpState* st = reinterpret_cast<OpState*>(reinterpret_cast<char*>(ldsBase) + sceneBytes + wave * perWave);
+  uint32_t* q = reinterpret_cast<uint32_t*>(st + p.coopOps);
+  uint32_t* shared = sharedAll + 4 * wave;
+  const uint32_t qmask = p.coopQueueCap - 1;
+  __syncthreads();  // the only workgroup barrier: the waves part here
+  const uint32_t listed = __hip_atomic_load(p.queue + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const uint32_t count = listed < p.heavyCap ? listed : p.heavyCap;
+  // The loop body must not end in a branch on the lane index: the compiler
+  // may thread such a branch into the lane-0 fetch below and send lanes 1-63
+  // back to the readfirstlane without lane 0 (they would then read k = 0
+  // forever).  So every lane stores the ray's results (the same bytes), and a
+  // wave barrier keeps the fetch block from being duplicated.
+  for (;;) {
+    __builtin_amdgcn_wave_barrier();
+    uint32_t k = 0;
+    if (lane == 0) k = atomicAdd(p.queue + 3, 1u);
+    k = __builtin_amdgcn_readfirstlane(k);
+    if (k >= count) break;
+    const uint2 e = p.heavy[k];
+    const Job& jb = kBatch ? p.jobs[__builtin_amdgcn_readfirstlane(e.x)] : p.job;
+    const SceneView sv = kBatch ? SceneView{jb.sc.ops, jb.sc.primA, jb.sc.primB} : sv0;
+    const uint32_t out = __builtin_amdgcn_readfirstlane(e.y);
+    const uint32_t spp = static_cast<uint32_t>(jb.spp);
+    const uint32_t pixel = out / spp, sample = out - pixel * spp;
+    const int x = static_cast<int>(pixel % static_cast<uint32_t>(jb.width));
+    const int y = static_cast<int>(pixel / static_cast<uint32_t>(jb.width));
+    float ox = 0.5f, oy = 0.5f;
+    if (jb.spp > 1) {
+      ox = JitterUnit(JitterHash(jb.seed, pixel, sample, 0));
+      oy = JitterUnit(JitterHash(jb.seed, pixel, sample, 1));
+    }
+    const RayF r = MakeRay(jb, x, y, ox, oy);
+    Counts n{0, 0, 0, 0, 0, 0};
+    const Hit h = CoopTrace<kFast, kCount>(sv, r, st, q, qmask, table, shared, lane, n);
+    if (kCount && jb.counters) {
+      int32_t v[6] = {n.go, n.comp, n.box, n.sph, n.cyl, n.cub};
+      for (int c = 0; c < 6; c++) {
+        for (int off = 32; off > 0; off >>= 1) v[c] += __shfl_xor(v[c], off, 64);
+      }
+      int32_t* c = jb.counters + 6 * static_cast<size_t>(out);
+      for (int i = 0; i < 6; i++) c[i] = v[i];
+    }
+    ShadeAndStore(jb, sv, r, h, out);
+  }
+  // The last wave out adds the launch's heavy rays to the stream's running
+  // total (words 6-7) and returns the heavy-ray counters to zero.
+  if (lane == 0) {
+    const uint32_t finished = atomicAdd(p.queue + 4, 1u);
+    if (finished == p.coopWaves - 1) {
+      atomicAdd(reinterpret_cast<unsigned long long*>(p.queue + 6), static_cast<unsigned long long>(count));
+      atomicExch(p.queue + 2, 0u);
+      atomicExch(p.queue + 3, 0u);
+      atomicExch(p.queue + 4, 0u);
+    }
+  }
+}
+
 constexpr int kMaxDepth = 64;
+
 constexpr size_t kStaticLds = 256;  // the action table and alignment slack
 
 template <bool kFast, bool kCount, bool kLds, bool kBatch>
@@ -695,6 +1021,21 @@ hipError_t Launch(const RenderParams& p, const LaunchShape& s, hipStream_t strea
     if (e != hipSuccess) return e;
   }
   hipLaunchKernelGGL(fn, dim3(s.grid), dim3(s.block), s.lds, stream, p);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess || p.budget <= 0) return e;
+  // the heavy rays the first kernel handed over
+  auto coop = CoopKernel<kFast, kCount, kBatch, false>;
+  if constexpr (!kBatch) {
+    if (p.coopLds) coop = CoopKernel<kFast, kCount, false, true>;
+  }
+  const size_t lds = CoopLaunchLds(p);
+  if (lds > 64 * 1024) {
+    e = hipFuncSetAttribute(reinterpret_cast<const void*>(coop), hipFuncAttributeMaxDynamicSharedMemorySize,
+                            static_cast<int>(lds));
+    if (e != hipSuccess) return e;
+  }
+  const uint32_t wavesPerGroup = static_cast<uint32_t>(p.coopBlock / 64);
+  hipLaunchKernelGGL(coop, dim3(p.coopWaves / wavesPerGroup), dim3(p.coopBlock), lds, stream, p);
   return hipGetLastError();
 }
 
@@ -727,6 +1068,26 @@ const DeviceLimits* CurrentLimits() {
 }  // namespace
 
 int32_t MaxSupportedDepth() { return kMaxDepth; }
+
+size_t CoopWaveLds(int32_t ops, uint32_t queueCap) {
+  return static_cast<size_t>(ops) * sizeof(OpState) + static_cast<size_t>(queueCap) * 4;
+}
+
+size_t CoopLaunchLds(const RenderParams& p) {
+  const size_t scene =
+      p.coopLds ? (static_cast<size_t>(p.job.sc.nOps) * 4 + static_cast<size_t>(p.job.sc.nPrims) * 2) * 16 : 0;
+  return scene + static_cast<size_t>(p.coopBlock / 64) * CoopWaveLds(p.coopOps, p.coopQueueCap);
+}
+
+size_t CoopLdsBytes(int32_t ops, uint32_t* queueCap) {
+  uint32_t cap = 1;
+  while (cap < static_cast<uint32_t>(ops > 0 ? ops : 1)) cap <<= 1;
+  *queueCap = cap;
+  const size_t bytes = CoopWaveLds(ops, cap);
+  const DeviceLimits* l = CurrentLimits();
+  const size_t limit = l && l->ldsPerBlock ? l->ldsPerBlock : 64 * 1024;
+  return bytes + kStaticLds <= limit ? bytes : 0;
+}
 
 hipError_t DeviceShapeLimits(int* computeUnits, size_t* ldsPerBlock) {
   const DeviceLimits* l = CurrentLimits();

@@ -36,8 +36,17 @@ csg_status BuildJob(const csg_scene* s, const csg_camera* cam, const csg_light* 
                     int32_t width, int32_t height, const csg_outputs* out, const csg_render_opts* opts, int dev,
                     int64_t tileBase, Job* jb);
 
-// The tile queue (two words, zero between launches) of (device, stream).
+// The tile queue (five words, zero between launches) of (device, stream).
 csg_status QueueFor(int dev, hipStream_t stream, uint32_t** out);
+
+// The heavy-ray list of (device, stream).
+csg_status HeavyFor(int dev, hipStream_t stream, uint2** out);
+
+// Fills the cooperative-evaluation fields of `p` for a launch whose scenes
+// have at most `maxOps` operators (deferral off when the options say so or
+// the cooperative kernel's state does not fit LDS).  `sceneLds` is the
+// scene's size when a plain launch stages it in LDS, else 0.
+void PrepareCoop(const LaunchConfig& cfg, int32_t maxOps, size_t sceneLds, uint2* heavy, RenderParams* p);
 
 // Copies a job table to the (device, stream) table buffer in stream order.
 csg_status UploadJobs(int dev, hipStream_t stream, const Job* jobs, size_t n, const Job** out);

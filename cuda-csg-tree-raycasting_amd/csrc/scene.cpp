@@ -258,7 +258,10 @@ FlatScene Flatten(const CSGTree& tree) {
     q[0] = F4{L.lo[0], L.lo[1], L.lo[2], R.lo[0]};
     q[1] = F4{L.hi[0], L.hi[1], L.hi[2], R.lo[1]};
     q[2] = F4{R.lo[2], R.hi[0], R.hi[1], R.hi[2]};
-    q[3] = F4{bits(desc(n.left)), bits(desc(n.right)), bits(static_cast<uint32_t>(n.type)), bits(parent)};
+    // type in bits 0-7; bit 8 set when this operator is its parent's right child
+    const bool isRight = n.parent >= 0 && tree.nodes[n.parent].right == i;
+    const uint32_t typeWord = static_cast<uint32_t>(n.type) | (isRight ? 0x100u : 0u);
+    q[3] = F4{bits(desc(n.left)), bits(desc(n.right)), bits(typeWord), bits(parent)};
   }
   f.primA.resize(f.nPrims);
   f.primB.resize(f.nPrims);

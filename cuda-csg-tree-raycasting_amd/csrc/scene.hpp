@@ -72,7 +72,8 @@ struct CSGTree {
 //   ops[4k+0] = (L.lo.x, L.lo.y, L.lo.z, R.lo.x)
 //   ops[4k+1] = (L.hi.x, L.hi.y, L.hi.z, R.lo.y)
 //   ops[4k+2] = (R.lo.z, R.hi.x, R.hi.y, R.hi.z)
-//   ops[4k+3] = bits(L.desc, R.desc, type, parent operator or 0xFFFFFFFF)
+//   ops[4k+3] = bits(L.desc, R.desc, type | right<<8, parent operator or 0xFFFFFFFF)
+//               (right = 1 when the operator is its parent's right child)
 //   child descriptor = node type | (index << 8), index = primitive index for a
 //   primitive child, operator index for an operator child
 //   primA[p] = (centre.xyz, radius or cube edge)

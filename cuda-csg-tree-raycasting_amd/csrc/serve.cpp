@@ -166,12 +166,17 @@ void Dispatch(csg_server* s, Slot& sl) {
   if (st == CSG_OK && csgray::ChooseShape(depth, 0, s->launch, &shape) != hipSuccess) st = CSG_E_HIP;
   uint32_t* queue = nullptr;
   if (st == CSG_OK) st = QueueFor(s->device, sl.stream, &queue);
+  uint2* heavy = nullptr;
+  if (st == CSG_OK) st = HeavyFor(s->device, sl.stream, &heavy);
   if (st == CSG_OK) {
     p.numJobs = static_cast<int32_t>(jobs.size());
     p.numTiles = static_cast<int32_t>(base);
     p.stackSlots = depth;
     p.queue = queue;
     p.totalWaves = static_cast<uint32_t>(shape.grid) * static_cast<uint32_t>(shape.block / 64);
+    int32_t maxOps = 1;
+    for (int64_t tk : take) maxOps = std::max(maxOps, s->scenes[s->reqs[tk].req.scene_id]->dev.nOps);
+    PrepareCoop(s->launch, maxOps, 0, heavy, &p);
     if (csgray::LaunchRender(p, s->launch, shape, sl.stream) != hipSuccess) st = CSG_E_HIP;
   }
   if (st == CSG_OK && hipEventRecord(sl.done, sl.stream) != hipSuccess) st = CSG_E_HIP;
@@ -223,6 +228,9 @@ csg_status csg_server_create(const csg_server_config* cfg, csg_server** out) {
   csg_render_opts o{};
   o.mode = s->cfg.mode;
   o.scene_placement = 1;
+  o.coop_budget = s->cfg.coop_budget;
+  o.coop_when = s->cfg.coop_when;
+  o.coop_grace = s->cfg.coop_grace;
   csg_status st = ParseOpts(&o, &s->launch);
   if (st == CSG_OK && hipGetDevice(&s->device) != hipSuccess) st = Fail(CSG_E_HIP, "hipGetDevice failed");
   s->slots.resize(static_cast<size_t>(s->cfg.streams));

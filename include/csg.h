@@ -114,6 +114,12 @@ typedef struct csg_render_opts {
                            p*n + s, each ray offset within its pixel by csg_jitter() */
   uint32_t seed;        /* jitter seed */
   int32_t tile_walk;    /* 0: row-major tiles; 1: Morton order within a job (square power-of-two grids) */
+  int32_t coop_budget;  /* state-machine steps a lane spends on one ray before the ray may be handed to a
+                           whole wave (cooperative evaluation, same results): 0 = default, < 0 = never */
+  int32_t coop_when;    /* 0: hand over only in the frame's tail, once every tile has been taken and a
+                           further coop_grace percent of the time that took has passed;
+                           1: as soon as the budget is spent */
+  int32_t coop_grace;   /* 0 = default (50); < 0 = none; else percent */
 } csg_render_opts;
 
 /* --- scene (replaces CSGTree::Parse + the upload in Raycaster::ChangeSize) --- */
@@ -149,6 +155,11 @@ csg_status csg_render_ex(const csg_scene* scene, const csg_camera* cam, const cs
                          const csg_phong* phong, int32_t width, int32_t height, const csg_outputs* out,
                          const csg_render_opts* opts, void* stream);
 csg_status csg_sync(void* stream);
+
+/* Rays handed to the cooperative kernel so far by csg_render, csg_render_ex
+ * and csg_render_batch launches on `stream` (current device); waits for the
+ * stream.  No reference counterpart: for tuning the hand-over and for tests. */
+csg_status csg_coop_rays(void* stream, int64_t* rays);
 
 /* Sub-pixel offset in [0,1) of sample `sample` of pixel `pixel` along axis 0
  * (x) or 1 (y): a 32-bit integer hash of (seed, pixel, sample, axis) with 24
@@ -205,6 +216,11 @@ typedef struct csg_launch_info {
   int32_t scene_in_lds;   /* 1 if the scene is staged in LDS */
   int32_t compute_units;
   int64_t lds_limit;      /* LDS per workgroup the device allows */
+  int32_t coop_budget;    /* steps before a ray is handed to the cooperative kernel (0: never) */
+  int32_t coop_grid;      /* workgroups of the cooperative kernel */
+  int32_t coop_block;     /* its threads per workgroup (64 per ray in flight) */
+  int32_t reserved0;
+  int64_t coop_lds_bytes; /* its dynamic LDS per workgroup (the scene copy included when staged) */
 } csg_launch_info;
 csg_status csg_render_plan(const csg_scene* scene, const csg_render_opts* opts, csg_launch_info* info);
 
@@ -220,6 +236,9 @@ typedef struct csg_server_config {
   int32_t streams;        /* batches in flight; 0 = 3 */
   int32_t max_batch_jobs; /* requests per batch; 0 = 256 */
   int32_t mode;           /* csg_mode */
+  int32_t coop_budget;    /* as csg_render_opts.coop_budget */
+  int32_t coop_when;      /* as csg_render_opts.coop_when */
+  int32_t coop_grace;     /* as csg_render_opts.coop_grace */
 } csg_server_config;
 typedef struct csg_request {
   int32_t scene_id;

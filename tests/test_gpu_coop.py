@@ -1,0 +1,192 @@
+"""Cooperative evaluation of heavy rays on the GPU.
+
+A ray whose state machine runs past the step budget is handed to a second
+kernel that evaluates it with a whole wave (independent subtrees at once).
+Every box test, primitive test and operator decision of the sequential
+machine happens exactly once there, so the outputs and the per-ray event
+counters must be bit-identical to the oracle and to the per-lane path,
+whatever the budget.  With eager hand-over (coop_when=1) budget 1 hands
+over every ray whose root is an operator and a budget of 16 splits a frame
+between the two kernels; by default (coop_when=0) only rays still running
+once every tile has been taken move, i.e. the frame's tail.
+"""
+import numpy as np
+import pytest
+import torch
+
+import csgray as C
+import oracle as O
+from common import setups
+from conftest import scene_names, scene_text
+from csgray import scenes as S
+
+pytestmark = pytest.mark.gpu
+
+
+def _np(d):
+    return {k: v.cpu().numpy() for k, v in d.items()}
+
+
+def _same(a, b, keys=("rgba", "depth", "prim_id", "flags")):
+    for k in keys:
+        assert np.array_equal(a[k].view(np.uint8), b[k].view(np.uint8)), k
+
+
+def _oracle_exact(text, cam6, l3, ph8, w, h, counters=True):
+    return O.OracleScene(text).render(cam6, l3, ph8, w, h, bits=32, counters=counters)
+
+
+@pytest.mark.parametrize("budget,when", [(1, 1), (16, 1), (1, 0)])
+@pytest.mark.parametrize("setup", [0, 1])
+@pytest.mark.parametrize("name", scene_names())
+def test_deferred_rays_match_fp32_oracle_bitwise(gpu, name, setup, budget, when):
+    print(f"\n[case] {name} setup={setup} budget={budget} when={when}", flush=True)
+    sname, cam, light, phong, cam6, l3, ph8 = setups()[setup]
+    w, h = 160, 90
+    text = scene_text(name)
+    s = C.Scene(text)
+    g = _np(s.render(cam, light, phong, w, h, mode=C.MODE_EXACT, counters=True, coop=budget, coop_when=when))
+    lane = _np(s.render(cam, light, phong, w, h, mode=C.MODE_EXACT, counters=True, coop=-1))
+    torch.cuda.synchronize()
+    s.close()
+    o = _oracle_exact(text, cam6, l3, ph8, w, h)
+    assert np.array_equal(g["prim_id"], o["prim_id"])
+    assert np.array_equal(g["flags"], o["flags"])
+    assert np.array_equal(g["depth"].view(np.uint32), o["depth"].view(np.uint32))
+    assert np.array_equal(g["counters"], o["counters"])
+    _same(g, lane, ("rgba", "depth", "prim_id", "flags", "counters"))
+
+
+def test_cheese_split_between_kernels_matches_oracle(gpu):
+    """The benchmark scene at 480x270 with the default budget and with 64
+    (tail-only and eager): heavy rays go cooperative, the rest stay per lane."""
+    print("\n[case] cheese 512 480x270 exact, budgets default/64, tail/eager", flush=True)
+    sname, cam, light, phong, cam6, l3, ph8 = setups()[1]
+    text = S.cheese(512, seed=512)
+    s = C.Scene(text)
+    o = _oracle_exact(text, cam6, l3, ph8, 480, 270)
+    for budget, when in ((0, 0), (64, 0), (64, 1)):
+        g = _np(s.render(cam, light, phong, 480, 270, mode=C.MODE_EXACT, counters=True, coop=budget, coop_when=when))
+        heavy = int(((g["counters"][..., 0] + g["counters"][..., 1]) > (budget or s.plan().coop_budget)).sum())
+        print(f"budget={budget} rays over budget={heavy}")
+        assert heavy > 0
+        assert np.array_equal(g["prim_id"], o["prim_id"])
+        assert np.array_equal(g["flags"], o["flags"])
+        assert np.array_equal(g["depth"].view(np.uint32), o["depth"].view(np.uint32))
+        assert np.array_equal(g["counters"], o["counters"])
+    s.close()
+
+
+def test_fast_mode_same_with_and_without_deferral(gpu):
+    """Fast mode, cheese 1920x1080: budgets 1, 64, default and never give the
+    same bits in all four outputs."""
+    print("\n[case] cheese 512 1920x1080 fast, budgets 1/64/default/never", flush=True)
+    sname, cam, light, phong, cam6, l3, ph8 = setups()[1]
+    s = C.Scene(S.cheese(512, seed=512))
+    ref = _np(s.render(cam, light, phong, 1920, 1080, coop=-1))
+    for budget, when in ((1, 1), (64, 1), (64, 0), (0, 0)):
+        _same(_np(s.render(cam, light, phong, 1920, 1080, coop=budget, coop_when=when)), ref)
+    s.close()
+
+
+def test_heavy_list_overflow_finishes_per_lane(gpu):
+    """2100x2100 rays with budget 1 overflow the 4 Mi-entry heavy list; the
+    rays past its end finish in the per-lane kernel with the same results."""
+    print("\n[case] Wikipedia 2100x2100 exact, budget 1 (heavy list overflow)", flush=True)
+    sname, cam, light, phong, cam6, l3, ph8 = setups()[1]
+    s = C.Scene(scene_text("testWikipedia.txt"))
+    a = _np(s.render(cam, light, phong, 2100, 2100, mode=C.MODE_EXACT, counters=True, coop=1, coop_when=1))
+    b = _np(s.render(cam, light, phong, 2100, 2100, mode=C.MODE_EXACT, counters=True, coop=-1))
+    # a second call on the same stream starts from a clean heavy list
+    c = _np(s.render(cam, light, phong, 2100, 2100, mode=C.MODE_EXACT, counters=True, coop=1, coop_when=1))
+    s.close()
+    _same(a, b, ("rgba", "depth", "prim_id", "flags", "counters"))
+    _same(c, b, ("rgba", "depth", "prim_id", "flags", "counters"))
+
+
+@pytest.mark.parametrize("name", ["testWikipedia.txt", "testCheese512.txt"])
+def test_deferred_jittered_samples_match_oracle(gpu, name):
+    print(f"\n[case] {name} 4 spp budget 1", flush=True)
+    sname, cam, light, phong, cam6, l3, ph8 = setups()[1]
+    text = scene_text(name)
+    s = C.Scene(text)
+    w, h, spp, seed = 64, 48, 4, 1234
+    g = _np(s.render(cam, light, phong, w, h, mode=C.MODE_EXACT, counters=True, samples=spp, seed=seed, coop=1,
+                     coop_when=1))
+    s.close()
+    o = O.OracleScene(text).render_samples(cam6, l3, ph8, w, h, spp, seed, bits=32, counters=True)
+    assert np.array_equal(g["prim_id"], o["prim_id"])
+    assert np.array_equal(g["depth"].view(np.uint32), o["depth"].view(np.uint32))
+    assert np.array_equal(g["flags"], o["flags"])
+    assert np.array_equal(g["counters"], o["counters"])
+
+
+def _batch_specs(n, seed, light, ph):
+    rng = np.random.default_rng(seed)
+    names = scene_names()
+    out = []
+    for i in range(n):
+        text = scene_text(names[i % len(names)]) if i % 2 else S.random_tree(rng)
+        eye = S.orbit_eye(rng, 5.0, 80.0)
+        w, h = int(rng.integers(8, 160)), int(rng.integers(8, 120))
+        sc = C.Scene(text)
+        out.append((text, sc, C.camera_look_at(eye, (0, 0, 0), 60), w, h))
+    return out
+
+
+def test_batch_with_deferral_matches_oracle(gpu):
+    print("\n[case] batch of 20 mixed jobs, exact, budget 1 and 16 eager, 1 tail", flush=True)
+    light, ph = C.light(S.LIGHT_TOWARD), C.phong(*S.PHONG)
+    specs = _batch_specs(20, 5, light, ph)
+    for budget, when in ((1, 1), (16, 1), (1, 0)):
+        full = [(sc, cam, light, ph, w, h, C.alloc_outputs(w, h)) for text, sc, cam, w, h in specs]
+        C.render_batch(C.make_jobs(full), mode=C.MODE_EXACT, coop=budget, coop_when=when)
+        torch.cuda.synchronize()
+        for (text, sc, cam, w, h), f in zip(specs, full):
+            g = _np(f[6])
+            o = O.OracleScene(text).render(C.cam_tuple(cam), S.LIGHT_TOWARD, O.phong8(*S.PHONG), w, h, bits=32)
+            assert np.array_equal(g["prim_id"], o["prim_id"])
+            assert np.array_equal(g["depth"].view(np.uint32), o["depth"].view(np.uint32))
+            assert np.array_equal(g["flags"], o["flags"])
+
+
+def test_captured_batch_with_deferral_replays_and_updates(gpu):
+    print("\n[case] captured batch, budget 1, replays and camera update", flush=True)
+    light, ph = C.light(S.LIGHT_TOWARD), C.phong(*S.PHONG)
+    specs = _batch_specs(12, 9, light, ph)
+    full = [(sc, cam, light, ph, w, h, C.alloc_outputs(w, h)) for text, sc, cam, w, h in specs]
+    want = [_np(sc.render(cam, light, ph, w, h, coop=-1)) for text, sc, cam, w, h in specs]
+    b = C.Batch(C.make_jobs(full), coop=1, coop_when=1)
+    for rep in range(3):
+        for f in full:
+            for t in f[6].values():
+                t.fill_(0)
+        torch.cuda.synchronize()
+        b.launch()
+        torch.cuda.synchronize()
+        _ = torch.ones(1000, device="cuda").sum().item()  # eager work between replays
+        for f, w_ in zip(full, want):
+            _same(_np(f[6]), w_)
+    # move every camera and resize; the graph's kernel nodes take the new jobs
+    rng = np.random.default_rng(3)
+    moved = []
+    for sc, cam, l_, p_, w, h, outs in full:
+        cam2 = C.camera_look_at(S.orbit_eye(rng, 4.0, 60.0), (0, 0, 0), 50)
+        w2, h2 = int(rng.integers(8, 90)), int(rng.integers(8, 90))
+        moved.append((sc, cam2, light, ph, w2, h2, C.alloc_outputs(w2, h2)))
+    b.update(C.make_jobs(moved))
+    b.launch()
+    torch.cuda.synchronize()
+    for sc, cam, l_, p_, w, h, outs in moved:
+        _same(_np(outs), _np(sc.render(cam, light, ph, w, h, coop=-1)))
+    b.close()
+
+
+def test_plan_reports_cooperative_kernel(gpu):
+    s = C.Scene(S.cheese(512, seed=512))
+    p = s.plan()
+    print(f"\ncoop budget={p.coop_budget} grid={p.coop_grid} lds={p.coop_lds_bytes}")
+    assert p.coop_budget > 0 and p.coop_grid >= p.compute_units and p.coop_lds_bytes >= 511 * 24
+    assert s.plan(coop=-1).coop_budget == 0
+    assert s.plan(coop=7).coop_budget == 7
+    s.close()
