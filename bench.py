@@ -97,6 +97,42 @@ def bench_batched(torch, C, S, reps, coop):
             "ms": {"plain": round(plain_ms, 2), "batched": round(batch_ms, 2), "graph": round(graph_ms, 2)}}
 
 
+def bench_stress(torch, C, S, reps, coop):
+    """Cube minus a balanced Union of 4095 small spheres / cylinders (8191
+    nodes, 13 operator levels, scene in global memory), 3840x2160 with 4
+    jittered samples per pixel: the plain launch against the captured batch
+    of the same frame after autotuning its tiling; both must give the same
+    bits."""
+    import numpy as np
+    w, h, spp, seed = 3840, 2160, 4, 4095
+    scene = C.Scene(S.stress(4095, seed=4095))
+    cam = C.camera_look_at(S.VIEW_EYE, S.VIEW_TARGET, S.VIEW_VFOV_DEG)
+    light, ph = C.light(S.LIGHT_TOWARD), C.phong(*S.PHONG)
+    outs = C.alloc_outputs(w, h, spp)
+    before = C.coop_rays()
+    plain_ms = _time_ms(torch, lambda: scene.render(cam, light, ph, w, h, outputs=outs, samples=spp, seed=seed,
+                                                    **coop), reps)
+    handed = (C.coop_rays() - before) / (reps + 1)
+    plain_img = {k: v.cpu().numpy() for k, v in outs.items()}
+    bouts = C.alloc_outputs(w, h, spp)
+    jobs = C.make_jobs([(scene, cam, light, ph, w, h, bouts)])
+    batch = C.Batch(jobs, samples=spp, seed=seed, **coop)
+    tune = batch.autotune(reps=1)
+    graph_ms = _time_ms(torch, batch.launch, reps)
+    same = all(np.array_equal(plain_img[k].view(np.uint8), bouts[k].cpu().numpy().view(np.uint8))
+               for k in ("rgba", "depth", "prim_id", "flags"))
+    batch.close()
+    rays = w * h * spp
+    info = scene.info()
+    plan = scene.plan(coop=coop["coop"])
+    return {"workload": "stress: cube minus Union of 4095 spheres/cylinders (8191 nodes), 3840x2160, 4 spp",
+            "rays": rays, "nodes": info.nodes, "op_depth": info.op_depth, "scene_in_lds": bool(plan.scene_in_lds),
+            "plain_ms": round(plain_ms, 2), "plain_mrays_s": round(rays / plain_ms / 1e3, 1),
+            "graph_ms": round(graph_ms, 2), "graph_mrays_s": round(rays / graph_ms / 1e3, 1),
+            "autotune": {"block": tune.block, "tile_walk": tune.tile_walk, "heaviest_first": tune.heaviest_first},
+            "coop_rays_per_frame": round(handed), "plain_equals_batched": same}
+
+
 def bench_serving(C, S, lambdas, seconds, coop):
     """Poisson arrivals of (tree, orbit camera, size) requests against 64
     resident trees; latency = submit -> result ready."""
@@ -173,7 +209,7 @@ def main():
                     help="0: hand rays over only once every tile is taken; 1: as soon as the budget is spent")
     ap.add_argument("--coop-grace", type=int, default=0,
                     help="tail hand-over waits this percent of the tile hand-out time (0 default, <0 none)")
-    ap.add_argument("--only", choices=("plain", "batched", "serving"), default=None,
+    ap.add_argument("--only", choices=("plain", "batched", "serving", "stress"), default=None,
                     help="run one leg (plain still sets the headline value)")
     args = ap.parse_args()
     coop = {"coop": args.coop_budget, "coop_when": args.coop_when, "coop_grace": args.coop_grace}
@@ -243,6 +279,8 @@ def main():
     if not args.no_extra:
         if args.only in (None, "batched"):
             extra["batched"] = bench_batched(torch, C, S, 3, coop)
+        if args.only in (None, "stress"):
+            extra["stress"] = bench_stress(torch, C, S, 2, coop)
         if args.only in (None, "serving"):
             extra["serving"] = bench_serving(C, S, (200, 1000, 5000), args.serve_seconds, coop)
 
