@@ -130,6 +130,14 @@ _SIGNATURES = [
                                      ctypes.POINTER(RenderOpts), _P]),
     ("csg_sync", ctypes.c_int, [_P]),
     ("csg_coop_rays", ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int64)]),
+    ("csg_camera_move", ctypes.c_int, [ctypes.POINTER(Camera), ctypes.c_float, ctypes.c_float, ctypes.c_float]),
+    ("csg_camera_rotate", ctypes.c_int, [ctypes.POINTER(Camera), ctypes.c_float, ctypes.c_float]),
+    ("csg_camera_input", ctypes.c_int, [ctypes.POINTER(Camera), ctypes.c_uint32, ctypes.c_int32, ctypes.c_float,
+                                         ctypes.c_float]),
+    ("csg_write_ppm", ctypes.c_int, [ctypes.c_char_p, _P, ctypes.c_int32, ctypes.c_int32]),
+    ("csg_write_png", ctypes.c_int, [ctypes.c_char_p, _P, ctypes.c_int32, ctypes.c_int32]),
+    ("csg_write_raw", ctypes.c_int, [ctypes.c_char_p, _P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                     ctypes.c_int32]),
     ("csg_render_plan", ctypes.c_int, [_P, ctypes.POINTER(RenderOpts), ctypes.POINTER(LaunchInfo)]),
     ("csg_jitter", ctypes.c_float, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32]),
     ("csg_render_batch", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(RenderOpts), _P]),
@@ -330,6 +338,53 @@ def alloc_outputs(width, height, spp=1, device=None, counters=False):
 def _outputs_struct(outputs):
     return Outputs(*(outputs[k].data_ptr() if outputs.get(k) is not None else None
                      for k in ("rgba", "depth", "prim_id", "flags")))
+
+
+KEY_FORWARD, KEY_BACKWARD, KEY_LEFT, KEY_RIGHT, KEY_UP, KEY_DOWN = 1, 2, 4, 8, 16, 32
+
+
+def camera_move(cam, forward, right, up):
+    """Camera::move in place (forward/right along the view, up along world Y)."""
+    _check(lib().csg_camera_move(ctypes.byref(cam), forward, right, up))
+    return cam
+
+
+def camera_rotate(cam, dpitch, dyaw):
+    """Camera::rotate in place (pitch clamped to +-89 degrees)."""
+    _check(lib().csg_camera_rotate(ctypes.byref(cam), dpitch, dyaw))
+    return cam
+
+
+def camera_input(cam, keys=0, mouse=None):
+    """One frame of the interactive input: KEY_* bits step 0.1; mouse=(dx, dy) rotates by -0.005/pixel."""
+    dx, dy = mouse if mouse is not None else (0.0, 0.0)
+    _check(lib().csg_camera_input(ctypes.byref(cam), int(keys), 1 if mouse is not None else 0, dx, dy))
+    return cam
+
+
+def _host(a, dtype):
+    import numpy as np
+    if hasattr(a, "detach"):
+        a = a.detach().cpu().numpy()
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+def write_image(path, rgba):
+    """Write an (H, W, 4) float RGBA render (row 0 = bottom) as .png or .ppm, top row first."""
+    import numpy as np
+    a = _host(rgba, np.float32)
+    h, w = a.shape[0], a.shape[1]
+    fn = lib().csg_write_png if str(path).lower().endswith(".png") else lib().csg_write_ppm
+    _check(fn(str(path).encode(), a.ctypes.data, w, h))
+
+
+def write_raw(path, arr, flip_rows=False):
+    """Write an (H, W[, ...]) array (depth f32, prim id i32, flags u8, ...) as raw little-endian bytes."""
+    import numpy as np
+    a = np.ascontiguousarray(arr.detach().cpu().numpy() if hasattr(arr, "detach") else arr)
+    h, w = a.shape[0], a.shape[1]
+    _check(lib().csg_write_raw(str(path).encode(), a.ctypes.data, a.itemsize * (a.size // (h * w)), w, h,
+                               1 if flip_rows else 0))
 
 
 def coop_rays(stream=None):

@@ -147,6 +147,33 @@ void csg_light_default(csg_light* light);           /* polar -60 deg, azimuth -4
 void csg_light_from_angles(float polar, float azimuth, csg_light* light);
 void csg_phong_default(csg_phong* phong);
 
+/* Camera control of the interactive application, for scripted fly-throughs.
+ * csg_camera_move: Camera::move (Camera.h:37-42), along the current forward
+ * and right vectors, `up` along world Y.  csg_camera_rotate: Camera::rotate
+ * (Camera.h:44-47), pitch clamped to +-89 degrees as in setRotation.
+ * csg_camera_input: one frame of Application::Input (Application.cpp:85-119):
+ * keys step 0.1 (forward/backward, right/left, up/down); with the mouse
+ * pressed, yaw -= 0.005 * dx and pitch -= 0.005 * dy, applied before the move. */
+enum {
+  CSG_KEY_FORWARD = 1, CSG_KEY_BACKWARD = 2, CSG_KEY_LEFT = 4, CSG_KEY_RIGHT = 8,
+  CSG_KEY_UP = 16, CSG_KEY_DOWN = 32
+};
+csg_status csg_camera_move(csg_camera* cam, float forward, float right, float up);
+csg_status csg_camera_rotate(csg_camera* cam, float dpitch, float dyaw);
+csg_status csg_camera_input(csg_camera* cam, uint32_t keys, int32_t mouse_pressed, float mouse_dx, float mouse_dy);
+
+/* Headless output (no reference counterpart: the reference only displays
+ * through OpenGL).  Inputs are HOST buffers in render order (row 0 = the
+ * bottom scanline); image files are written top row first, i.e. rows are
+ * flipped.  Channels map [0,1] to 0..255 (round to nearest, clamped, NaN 0).
+ * csg_write_ppm: binary P6 (RGB).  csg_write_png: 8-bit RGBA, stored
+ * (uncompressed) deflate blocks.  csg_write_raw: the array as is (little
+ * endian, `elem_bytes` per pixel), rows flipped when `flip_rows`. */
+csg_status csg_write_ppm(const char* path, const float* rgba, int32_t width, int32_t height);
+csg_status csg_write_png(const char* path, const float* rgba, int32_t width, int32_t height);
+csg_status csg_write_raw(const char* path, const void* data, int32_t elem_bytes, int32_t width, int32_t height,
+                         int32_t flip_rows);
+
 /* --- render (replaces Raycaster::Raycast: RaycastKernel + LightningKernel) --- */
 csg_status csg_render(const csg_scene* scene, const csg_camera* cam, const csg_light* light,
                       const csg_phong* phong, int32_t width, int32_t height, const csg_outputs* out,
