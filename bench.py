@@ -21,7 +21,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "cuda-csg-tree-raycasting_amd"))
 
 FLOP_FIXTURE = os.path.join(ROOT, "tests", "golden", "events_cheese512_1080p.json")
-TRAFFIC_FIXTURE = os.path.join(ROOT, "profiles", "traffic_render_cheese512_1080p.json")
+# HBM bytes per render call from the FETCH_SIZE / WRITE_SIZE counter runs,
+# scaled by the same counters on a copy of known size (line
+# "hbm_bytes_per_launch: N" of this summary).
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "traffic_render_cheese512_1080p.md")
+COPY_MIB = 256
 
 # FP32 VALU operations per event, counted on the reference lines the event
 # executes: ray generation (RaycastingKernels.cu:11-27, Ray.cuh:12-18), box test
@@ -95,6 +99,20 @@ def bench_batched(torch, C, S, reps, coop):
             "batched_vs_plain": round(plain_ms / min(batch_ms, graph_ms), 2),
             "autotune": {"block": tune.block, "tile_walk": tune.tile_walk, "heaviest_first": tune.heaviest_first},
             "ms": {"plain": round(plain_ms, 2), "batched": round(batch_ms, 2), "graph": round(graph_ms, 2)}}
+
+
+def bench_wikipedia(torch, C, S, reps, coop):
+    """The Wikipedia scene (Difference of an Intersection of cube and sphere
+    with a Union of three cylinders) at 640x360 from (3, 2.5, 4): the
+    reference's own CPU-runnable configuration, on the GPU."""
+    w, h = 640, 360
+    scene = C.Scene(S.WIKIPEDIA)
+    cam = C.camera_look_at(S.VIEW_EYE, S.VIEW_TARGET, S.VIEW_VFOV_DEG)
+    light, ph = C.light(S.LIGHT_TOWARD), C.phong(*S.PHONG)
+    outs = C.alloc_outputs(w, h)
+    ms = _time_ms(torch, lambda: scene.render(cam, light, ph, w, h, outputs=outs, **coop), reps)
+    return {"workload": "Wikipedia scene, 640x360, 1 ray/pixel", "rays": w * h, "ms": round(ms, 4),
+            "mrays_s": round(w * h / ms / 1e3, 1)}
 
 
 def bench_stress(torch, C, S, reps, coop):
@@ -193,6 +211,20 @@ def bench_serving(C, S, lambdas, seconds, coop):
     return out
 
 
+def calibrate_copy():
+    """A plain device copy of COPY_MIB MiB, five times: each dispatch reads and
+    writes exactly COPY_MIB MiB, which scales the FETCH_SIZE / WRITE_SIZE
+    counters of the render's profile."""
+    import torch
+    n = COPY_MIB * (1 << 20) // 4
+    a = torch.arange(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_mib": COPY_MIB, "dispatches": 5}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -209,10 +241,15 @@ def main():
                     help="0: hand rays over only once every tile is taken; 1: as soon as the budget is spent")
     ap.add_argument("--coop-grace", type=int, default=0,
                     help="tail hand-over waits this percent of the tile hand-out time (0 default, <0 none)")
-    ap.add_argument("--only", choices=("plain", "batched", "serving", "stress"), default=None,
+    ap.add_argument("--only", choices=("plain", "wikipedia", "batched", "serving", "stress"), default=None,
                     help="run one leg (plain still sets the headline value)")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help=f"only copy a {COPY_MIB} MiB buffer 5 times (reference for the HBM byte counters)")
     args = ap.parse_args()
     coop = {"coop": args.coop_budget, "coop_when": args.coop_when, "coop_grace": args.coop_grace}
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
         sys.exit(2)
@@ -267,18 +304,22 @@ def main():
         fpr = flops_per_ray(ev)
         achieved = fpr * rays / (avg_kernel_ms / 1e3) / 1e12
         traffic = None
-        if os.path.exists(TRAFFIC_FIXTURE):
-            with open(TRAFFIC_FIXTURE) as f:
-                traffic = json.load(f).get("hbm_bytes_per_launch")
+        if os.path.exists(TRAFFIC_SUMMARY):
+            with open(TRAFFIC_SUMMARY) as f:
+                for ln in f:
+                    if ln.startswith("hbm_bytes_per_launch:"):
+                        traffic = int(ln.split(":")[1])
         roofline = {"bound": "valu_fp32", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
                     "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": traffic,
-                    "flops_per_ray": round(fpr, 1), "kernel": "RenderKernel<fast>",
+                    "flops_per_ray": round(fpr, 1), "kernel": "render call (RenderKernel + CoopKernel)",
                     "avg_kernel_ms": round(avg_kernel_ms, 4)}
 
     extra = {}
     if not args.no_extra:
         if args.only in (None, "batched"):
             extra["batched"] = bench_batched(torch, C, S, 3, coop)
+        if args.only in (None, "wikipedia"):
+            extra["wikipedia"] = bench_wikipedia(torch, C, S, 200, coop)
         if args.only in (None, "stress"):
             extra["stress"] = bench_stress(torch, C, S, 2, coop)
         if args.only in (None, "serving"):

@@ -69,6 +69,38 @@ def compare_fp64(g, o64, rows=None):
     return rate, (float(rel.max()) if rel.size else 0.0), float(1 - unflagged.sum() / max(sel.sum(), 1))
 
 
+def accuracy_report(label, g, o64, rows=None):
+    """Prints the per-configuration accuracy figures: hit-id agreement on all
+    and on unflagged rays, inside/outside (hit vs miss) agreement, flagged
+    fraction, and depth relative error max / p99.9 on agreeing hits."""
+    sel = np.ones(o64["prim_id"].shape, bool)
+    if rows is not None:
+        sel = np.zeros(o64["prim_id"].shape, bool)
+        sel[rows[0]:rows[1]:rows[2]] = True
+    unf = (o64["degen"] == 0) & sel
+    same_id = g["prim_id"] == o64["prim_id"]
+    same_hit = (g["prim_id"] >= 0) == (o64["prim_id"] >= 0)
+    hit = same_id & sel & (o64["prim_id"] >= 0)
+    rel = np.abs(g["depth"][hit].astype(np.float64) - o64["depth"][hit]) / np.abs(o64["depth"][hit])
+    n = int(sel.sum())
+    print(f"[accuracy] {label}: rays={n} id_agree_all={same_id[sel].mean():.6f} "
+          f"id_agree_unflagged={same_id[unf].mean():.6f} inside_outside_agree={same_hit[sel].mean():.6f} "
+          f"flagged={1 - unf.sum() / n:.4%} depth_rel_max={rel.max() if rel.size else 0:.2e} "
+          f"depth_rel_p999={np.quantile(rel, 0.999) if rel.size else 0:.2e}", flush=True)
+
+
+def test_fast_vs_fp64_wikipedia_full_size(gpu):
+    """The Wikipedia configuration (640x360 from (3, 2.5, 4)) in fast mode against FP64."""
+    _label("testWikipedia.txt 640x360 fast", 1)
+    sname, cam, light, phong, cam6, l3, ph8 = setups()[1]
+    text = scene_text("testWikipedia.txt")
+    g = gpu_render(text, cam, light, phong, 640, 360, C.MODE_FAST)
+    o64 = O.OracleScene(text).render(cam6, l3, ph8, 640, 360, bits=64, degen=True)
+    accuracy_report("wikipedia 640x360 fast", g, o64)
+    rate, drel, flagged = compare_fp64(g, o64)
+    assert rate >= AGREE_MIN and drel <= DEPTH_REL_TOL
+
+
 @pytest.mark.parametrize("setup", [0, 1])
 @pytest.mark.parametrize("name", scene_names())
 def test_fast_vs_fp64_oracle(gpu, name, setup):
@@ -93,6 +125,7 @@ def test_exact_vs_fp64_wikipedia_full_size(gpu):
     o32 = O.OracleScene(text).render(cam6, l3, ph8, 640, 360, bits=32)
     assert np.array_equal(g["prim_id"], o32["prim_id"])
     o64 = O.OracleScene(text).render(cam6, l3, ph8, 640, 360, bits=64, degen=True)
+    accuracy_report("wikipedia 640x360 exact", g, o64)
     rate, drel, flagged = compare_fp64(g, o64)
     print(f"agree(unflagged)={rate:.6f} depth_rel_max={drel:.2e} flagged={flagged:.4%}")
     assert rate >= AGREE_MIN and drel <= DEPTH_REL_TOL
@@ -108,6 +141,7 @@ def test_fast_vs_fp64_cheese512_spheres_1080p(gpu):
     g = gpu_render(text, cam, light, phong, 1920, 1080, C.MODE_FAST)
     rows = (0, 1080, 4)
     o64 = O.OracleScene(text).render(cam6, l3, ph8, 1920, 1080, bits=64, degen=True, rows=rows)
+    accuracy_report("cheese 1920x1080 fast (every 4th row)", g, o64, rows=rows)
     rate, drel, flagged = compare_fp64(g, o64, rows=rows)
     print(f"agree(unflagged)={rate:.6f} depth_rel_max={drel:.2e} flagged={flagged:.4%}")
     assert rate >= AGREE_MIN and drel <= DEPTH_REL_TOL
