@@ -182,6 +182,11 @@ csg_status csg_render_ex(const csg_scene* scene, const csg_camera* cam, const cs
                          const csg_phong* phong, int32_t width, int32_t height, const csg_outputs* out,
                          const csg_render_opts* opts, void* stream);
 csg_status csg_sync(void* stream);
+/* Each (device, stream) pair gets a small workspace (tile queue, heavy-ray
+ * list) on its first render, cleared with hipMemsetAsync on that stream; the
+ * kernels leave it cleared after every launch.  To capture csg_render /
+ * csg_render_ex / csg_render_batch into a graph, render once on the stream
+ * first, so that no memset is recorded. */
 
 /* Rays handed to the cooperative kernel so far by csg_render, csg_render_ex
  * and csg_render_batch launches on `stream` (current device); waits for the
