@@ -117,7 +117,7 @@ csg_status Stage(csg_batch* b) {
     const hipError_t e = csgray::ChooseShape(depth, 0, b->cfg, &ch.shape);
     if (e != hipSuccess) return FailHip(e, "choosing the launch shape");
     p.totalWaves = static_cast<uint32_t>(ch.shape.grid) * static_cast<uint32_t>(ch.shape.block / 64);
-    PrepareCoop(b->cfg, maxOps, 0, ch.heavy, &p);
+    PrepareCoop(b->cfg, maxOps, ch.heavy, &p);
   }
   return CSG_OK;
 }
@@ -229,8 +229,8 @@ csg_status PatchNodes(csg_batch* b) {
       void* cargs[1] = {&ch.params};
       cp.kernelParams = cargs;
       cp.extra = nullptr;
-      cp.gridDim = dim3(ch.params.coopWaves / static_cast<uint32_t>(ch.params.coopBlock / 64));
-      cp.blockDim = dim3(ch.params.coopBlock);
+      cp.gridDim = dim3(ch.params.coopWaves);
+      cp.blockDim = dim3(64);
       cp.sharedMemBytes = static_cast<unsigned int>(csgray::CoopLaunchLds(ch.params));
       e = hipGraphExecKernelNodeSetParams(b->exec, ch.coopNode, &cp);
       if (e != hipSuccess) return FailHip(e, "hipGraphExecKernelNodeSetParams");

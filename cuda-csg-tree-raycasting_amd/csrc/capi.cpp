@@ -119,19 +119,16 @@ csg_status HeavyFor(int dev, hipStream_t stream, uint2** out) {
 }
 
 // Budget (0: no deferral), grid and LDS of the cooperative kernel for scenes
-// of at most `maxOps` operators.  `sceneLds` > 0 (plain launches whose render
-// stages the scene in LDS) lets the cooperative workgroups stage it too, with
-// as many one-ray waves per workgroup as the rest of the LDS holds.
+// of at most `maxOps` operators: one-wave workgroups over the scene in global
+// memory, up to 16 per CU as their LDS allows.
 struct CoopPlan {
   int32_t budget = 0;
   uint32_t waves = 0;
   uint32_t queueCap = 0;
-  int32_t block = 64;
-  bool lds = false;
   size_t ldsBytes = 0;
 };
 
-CoopPlan PlanCoop(const LaunchConfig& cfg, int32_t maxOps, size_t sceneLds) {
+CoopPlan PlanCoop(const LaunchConfig& cfg, int32_t maxOps) {
   CoopPlan c;
   if (cfg.budget < 0 || maxOps <= 0) return c;
   const size_t wave = CoopLdsBytes(maxOps, &c.queueCap);
@@ -139,29 +136,18 @@ CoopPlan PlanCoop(const LaunchConfig& cfg, int32_t maxOps, size_t sceneLds) {
   int cus = 0;
   size_t ldsPerBlock = 0;
   if (DeviceShapeLimits(&cus, &ldsPerBlock) != hipSuccess || cus <= 0) return c;
-  const size_t limit = (ldsPerBlock ? ldsPerBlock : 65536) - 1024;  // static LDS and slack
   const size_t ldsPerCu = 160 * 1024;
+  const size_t perCu = std::max<size_t>(1, std::min<size_t>(16, ldsPerCu / (wave + 256)));
   c.budget = cfg.budget > 0 ? cfg.budget : kDefaultBudget;
-  if (sceneLds > 0 && sceneLds + wave <= limit) {
-    // one workgroup per CU holding the scene and up to 16 waves
-    const size_t w = std::min<size_t>(16, (limit - sceneLds) / wave);
-    c.lds = true;
-    c.block = static_cast<int32_t>(64 * w);
-    c.ldsBytes = sceneLds + w * wave;
-    c.waves = static_cast<uint32_t>(static_cast<size_t>(cus) * w);
-  } else {
-    // one-wave workgroups over the scene in global memory, up to 16 per CU
-    const size_t perCu = std::max<size_t>(1, std::min<size_t>(16, ldsPerCu / (wave + 256)));
-    c.waves = static_cast<uint32_t>(static_cast<size_t>(cus) * perCu);
-    c.ldsBytes = wave;
-  }
+  c.waves = static_cast<uint32_t>(static_cast<size_t>(cus) * perCu);
+  c.ldsBytes = wave;
   return c;
 }
 
-void PrepareCoop(const LaunchConfig& cfg, int32_t maxOps, size_t sceneLds, uint2* heavy, RenderParams* p) {
+void PrepareCoop(const LaunchConfig& cfg, int32_t maxOps, uint2* heavy, RenderParams* p) {
   p->budget = 0;
   if (!heavy) return;
-  const CoopPlan c = PlanCoop(cfg, maxOps, sceneLds);
+  const CoopPlan c = PlanCoop(cfg, maxOps);
   if (c.budget <= 0) return;
   p->budget = c.budget;
   p->heavy = heavy;
@@ -169,8 +155,6 @@ void PrepareCoop(const LaunchConfig& cfg, int32_t maxOps, size_t sceneLds, uint2
   p->coopOps = maxOps;
   p->coopQueueCap = c.queueCap;
   p->coopWaves = c.waves;
-  p->coopBlock = c.block;
-  p->coopLds = c.lds ? 1 : 0;
   p->coopEager = cfg.coopEager ? 1 : 0;
   p->coopGrace = cfg.coopGrace > 0 ? cfg.coopGrace : (cfg.coopGrace < 0 ? 0 : kDefaultCoopGrace);
 }
@@ -518,10 +502,10 @@ csg_status csg_render_plan(const csg_scene* s, const csg_render_opts* opts, csg_
   info->scene_in_lds = shape.sceneInLds ? 1 : 0;
   info->compute_units = cus;
   info->lds_limit = static_cast<int64_t>(ldsLimit);
-  const CoopPlan c = PlanCoop(cfg, s->dev.nOps, shape.sceneInLds ? SceneBytes(s) : 0);
+  const CoopPlan c = PlanCoop(cfg, s->dev.nOps);
   info->coop_budget = c.budget;
-  info->coop_grid = static_cast<int32_t>(c.waves / static_cast<uint32_t>(c.block / 64));
-  info->coop_block = c.block;
+  info->coop_grid = static_cast<int32_t>(c.waves);
+  info->coop_block = 64;
   info->coop_lds_bytes = static_cast<int64_t>(c.ldsBytes);
   return CSG_OK;
 }
@@ -579,7 +563,7 @@ csg_status csg_render_ex(const csg_scene* s, const csg_camera* cam, const csg_li
     const csg_status hsd = HeavyFor(dev, hs, &heavy);
     if (hsd != CSG_OK) return hsd;
   }
-  PrepareCoop(cfg, s->dev.nOps, shape.sceneInLds ? SceneBytes(s) : 0, heavy, &p);
+  PrepareCoop(cfg, s->dev.nOps, heavy, &p);
   e = csgray::LaunchRender(p, cfg, shape, hs);
   if (e != hipSuccess) return FailHip(e, "render launch");
   return CSG_OK;
@@ -649,7 +633,7 @@ csg_status csg_render_batch(const csg_job* jobs, int32_t njobs, const csg_render
     }
     int32_t maxOps = 1;
     for (int32_t i = 0; i < n; i++) maxOps = std::max(maxOps, jobs[start + i].scene->dev.nOps);
-    PrepareCoop(cfg, maxOps, 0, heavy, &p);
+    PrepareCoop(cfg, maxOps, heavy, &p);
     e = csgray::LaunchRender(p, cfg, shape, hs);
     if (e != hipSuccess) return FailHip(e, "batched render launch");
   }

@@ -68,8 +68,6 @@ struct RenderParams {
   int32_t coopOps;            // operator-state slots per wave (max operators of the launch)
   uint32_t coopQueueCap;      // task ring entries (power of two >= coopOps)
   uint32_t coopWaves;         // waves in the cooperative grid
-  int32_t coopBlock;          // threads per cooperative workgroup (a multiple of 64)
-  int32_t coopLds;            // 1: the cooperative workgroups stage the scene in LDS (plain launches)
 };
 // queue words: [0] next tile, [1] tile waves done, [2] heavy rays, [3] next heavy ray, [4] coop waves done,
 // [6..7] heavy rays handed over on this stream so far (64-bit); on their own cache line, [32..33] time
@@ -134,8 +132,18 @@ constexpr uint32_t kHeavyCap = 1u << 22;
 // LDS bytes the cooperative kernel needs for scenes of at most `ops`
 // operators (0 when it cannot run: the caller then disables deferral).
 size_t CoopLdsBytes(int32_t ops, uint32_t* queueCap);
-// LDS of one wave's cooperative state, and of one cooperative workgroup of `p`.
-size_t CoopWaveLds(int32_t ops, uint32_t queueCap);
+// LDS of one wave's cooperative state (one state per operator, the task
+// ring; rounded to 16 B), i.e. of one cooperative workgroup of `p`.
+#if defined(__HIPCC__)
+#define CSG_HOST_DEVICE __host__ __device__
+#else
+#define CSG_HOST_DEVICE
+#endif
+constexpr size_t kCoopOpStateBytes = 24;
+CSG_HOST_DEVICE inline size_t CoopWaveLds(int32_t ops, uint32_t queueCap) {
+  const size_t bytes = static_cast<size_t>(ops) * kCoopOpStateBytes + static_cast<size_t>(queueCap) * 4;
+  return (bytes + 15) & ~static_cast<size_t>(15);
+}
 size_t CoopLaunchLds(const RenderParams& p);
 
 // Largest operator depth the traversal stack supports.

@@ -86,6 +86,7 @@ struct Hit {
 constexpr uint32_t kAborted = 128;
 // Upper bound on state-machine iterations per ray.
 constexpr int kIterCap = 1 << 20;
+constexpr bool kFuseSteps = true;
 
 // Stack-slot action codes (bits 11-12).
 constexpr uint32_t kSlotLoadLft = 1u << 11, kSlotLoadRgh = 2u << 11, kSlotSaveLft = 3u << 11,
@@ -357,7 +358,8 @@ __device__ Hit Trace(const SceneView& sv, const RayF& r, uint32_t rootDesc, uint
   int iter = 0;
 
   while (!done) {
-    if (action <= kGotoRgh) {
+    const bool stepGo = action <= kGotoRgh;
+    if (stepGo) {
       if (kCount) n.go++;
       const bool left = action == kGotoLft;
       const uint32_t d = left ? lDesc : rDesc;
@@ -412,7 +414,11 @@ __device__ Hit Trace(const SceneView& sv, const RayF& r, uint32_t rootDesc, uint
         }
         action = dL ? kGotoLft : (dR ? kGotoRgh : kCompute);
       }
-    } else {
+    }
+    // With kFuseSteps, a GoTo that does not descend is followed by its
+    // Compute in the same iteration (the reference's loop groups them the
+    // same way); the sequence of steps is unchanged.
+    if (kFuseSteps ? action >= kCompute : !stepGo) {
       if (kCount) n.comp++;
       L = action == kLoadLft ? held : L;
       R = action == kLoadRgh ? held : R;
@@ -792,30 +798,46 @@ __device__ __forceinline__ void WaveSync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+static_assert(sizeof(OpState) == kCoopOpStateBytes, "cooperative LDS sizing assumes 24-byte operator states");
+
 __device__ __forceinline__ uint32_t EnterTask(uint32_t op) { return op << 1; }
 __device__ __forceinline__ uint32_t ComputeTask(uint32_t op) { return (op << 1) | 1u; }
 
+__device__ __forceinline__ uint32_t LanesBelow(uint64_t mask) {
+  return __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mask >> 32),
+                                   __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u));
+}
+
+// One iteration: every lane with a task runs it (ENTER: box tests, the hit
+// primitive children's tests, spawns; COMPUTE: the decision, delivery or
+// loop).  A lane keeps the first task it spawns for the next iteration and
+// queues a second one in the LDS ring, from which idle lanes take work; the
+// root's result is found with a ballot.
 template <bool kFast, bool kCount>
 __device__ Hit CoopTrace(const SceneView& sv, const RayF& r, OpState* st, uint32_t* q, uint32_t qmask,
-                         const uint16_t* table, uint32_t* shared, int lane, Counts& n) {
-  if (lane == 0) {
-    st[0].tmin = 0.0f;
-    q[0] = EnterTask(0);
-    shared[0] = 0;
-  }
-  uint32_t head = 0, tail = 1;
+                         const uint16_t* table, int lane, Counts& n) {
+  uint32_t mine = lane == 0 ? EnterTask(0) : kNone;
+  if (lane == 0) st[0].tmin = 0.0f;
+  uint32_t head = 0, tail = 0;
+  Hit res{-1.0f, kMissBit | kAborted | 0xFFFF0000u};
   WaveSync();
   for (int iter = 0; iter < kIterCap; iter++) {
-    const uint32_t avail = tail - head;
-    if (avail == 0) break;  // nothing alive and no result: a malformed tree
-    const uint32_t task = static_cast<uint32_t>(lane) < avail ? q[(head + lane) & qmask] : kNone;
-    head += avail < 64u ? avail : 64u;
+    // idle lanes take queued tasks in lane order
+    const uint64_t idle = __ballot(mine == kNone);
+    const uint32_t rank = LanesBelow(idle);
+    const uint32_t avail = tail - head, nIdle = static_cast<uint32_t>(__popcll(idle));
+    const uint32_t take = avail < nIdle ? avail : nIdle;
+    if (mine == kNone && rank < take) mine = q[(head + rank) & qmask];
+    head += take;
+    if (__ballot(mine != kNone) == 0) break;  // nothing alive and no result: a malformed tree
     uint32_t sp0 = kNone, sp1 = kNone;
-    if (task != kNone) {
-      const uint32_t X = task >> 1;
+    bool fin = false;
+    Hit out = MissRecord();
+    if (mine != kNone) {
+      const uint32_t X = mine >> 1;
       const float4 q3 = sv.ops[4 * X + 3];
       const uint32_t lDesc = AsU(q3.x), rDesc = AsU(q3.y);
-      if ((task & 1u) == 0) {
+      if ((mine & 1u) == 0) {
         if (kCount) n.go++;
         const float tmin = st[X].tmin;
         const float4 q0 = sv.ops[4 * X + 0];
@@ -838,25 +860,28 @@ __device__ Hit CoopTrace(const SceneView& sv, const RayF& r, OpState* st, uint32
           }
         }
         const bool dL = gL & !pL, dR = gR & !pR;
-        if (dL) {
-          st[DescIndex(lDesc)].tmin = tmin;
-          sp0 = EnterTask(DescIndex(lDesc));
-        } else {
+        if (!dL) {
           st[X].lt = L.t;
           st[X].lb = L.bits;
         }
-        if (dR) {
-          st[DescIndex(rDesc)].tmin = tmin;
-          sp1 = EnterTask(DescIndex(rDesc));
-        } else {
+        if (!dR) {
           st[X].rt = R.t;
           st[X].rb = R.bits;
+        }
+        if (dL) {
+          st[DescIndex(lDesc)].tmin = tmin;
+          sp0 = EnterTask(DescIndex(lDesc));
+        }
+        if (dR) {
+          st[DescIndex(rDesc)].tmin = tmin;
+          if (dL) sp1 = EnterTask(DescIndex(rDesc));
+          else sp0 = EnterTask(DescIndex(rDesc));
         }
         st[X].pending = static_cast<uint32_t>(dL) + static_cast<uint32_t>(dR);
         if (!dL & !dR) sp0 = ComputeTask(X);
       } else {
         if (kCount) n.comp++;
-        Hit L{st[X].lt, st[X].lb}, R{st[X].rt, st[X].rb};
+        const Hit L{st[X].lt, st[X].lb}, R{st[X].rt, st[X].rb};
         const uint32_t typeWord = AsU(q3.z), parent = AsU(q3.w);
         const uint32_t acts = table[(typeWord & 0xFFu) * 9u + State(L.bits) * 3u + State(R.bits)];
         const bool lc = L.t < R.t, rc = L.t > R.t;
@@ -890,71 +915,54 @@ __device__ Hit CoopTrace(const SceneView& sv, const RayF& r, OpState* st, uint32
           }
         } else {
           const uint32_t rb = (acts & kFlipR) ? (R.bits ^ (kFlip | kExit | kEnter)) : R.bits;
-          const Hit res = retL ? L : (retR ? Hit{R.t, rb} : MissRecord());
+          out = retL ? L : (retR ? Hit{R.t, rb} : MissRecord());
           if (parent == kNone) {
-            shared[1] = AsU(res.t);
-            shared[2] = res.bits;
-            shared[0] = 1;
-          } else if (typeWord & 0x100u) {
-            st[parent].rt = res.t;
-            st[parent].rb = res.bits;
+            fin = true;
           } else {
-            st[parent].lt = res.t;
-            st[parent].lb = res.bits;
+            if (typeWord & 0x100u) {
+              st[parent].rt = out.t;
+              st[parent].rb = out.bits;
+            } else {
+              st[parent].lt = out.t;
+              st[parent].lb = out.bits;
+            }
+            if (atomicSub(&st[parent].pending, 1u) == 1u) sp0 = ComputeTask(parent);
           }
-          if (parent != kNone && atomicSub(&st[parent].pending, 1u) == 1u) sp0 = ComputeTask(parent);
         }
       }
     }
-    // append the new tasks in lane order
-    const uint64_t b1 = __ballot(sp0 != kNone);
-    const uint64_t b2 = __ballot(sp1 != kNone);
-    const uint32_t below = __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(b1 >> 32),
-                                                     __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(b1), 0u)) +
-                           __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(b2 >> 32),
-                                                     __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(b2), 0u));
-    uint32_t at = tail + below;
-    if (sp0 != kNone) q[(at++) & qmask] = sp0;
-    if (sp1 != kNone) q[at & qmask] = sp1;
-    tail += static_cast<uint32_t>(__popcll(b1) + __popcll(b2));
+    const uint64_t finished = __ballot(fin);
+    if (finished != 0) {
+      const int src = __ffsll(static_cast<unsigned long long>(finished)) - 1;
+      res.t = __uint_as_float(__builtin_amdgcn_readlane(AsU(out.t), src));
+      res.bits = __builtin_amdgcn_readlane(out.bits, src);
+      break;
+    }
+    // keep the first spawned task, queue the second
+    mine = sp0;
+    const uint64_t bq = __ballot(sp1 != kNone);
+    if (sp1 != kNone) q[(tail + LanesBelow(bq)) & qmask] = sp1;
+    tail += static_cast<uint32_t>(__popcll(bq));
     WaveSync();
-    if (shared[0]) break;
   }
-  const Hit res = shared[0] ? Hit{__uint_as_float(shared[1]), shared[2]} : Hit{-1.0f, kMissBit | kAborted | 0xFFFF0000u};
   WaveSync();
   return res;
 }
 
-// Each wave of a workgroup evaluates its own heavy rays.  LDS: when kLds,
-// the scene copy shared by the workgroup (plain launches), then per wave the
-// operator states (coopOps) and the task ring (coopQueueCap).
-template <bool kFast, bool kCount, bool kBatch, bool kLds>
-__global__ void __launch_bounds__(1024) CoopKernel(RenderParams p) {
+// One wave per workgroup evaluates heavy rays one after another, reading
+// the scene from global memory (L2-resident): per-wave LDS holds only the
+// operator states and the task ring, so a CU keeps up to 16 rays in flight
+// (the scene copy in LDS would leave room for fewer, which measured slower).
+template <bool kFast, bool kCount, bool kBatch>
+__global__ void __launch_bounds__(64) CoopKernel(RenderParams p) {
   extern __shared__ uint4 ldsBase[];
   __shared__ uint16_t table[27];
-  __shared__ uint32_t sharedAll[16 * 4];
-  const int tid = threadIdx.x;
-  const int lane = tid & 63, wave = tid >> 6;
-  if (tid < 27) table[tid] = kActionTable[tid];
-  SceneView sv0{p.job.sc.ops, p.job.sc.primA, p.job.sc.primB};
-  size_t sceneBytes = 0;
-  if (kLds) {
-    float4* lds = reinterpret_cast<float4*>(ldsBase);
-    const int nOps4 = 4 * p.job.sc.nOps, nP = p.job.sc.nPrims;
-    for (int i = tid; i < nOps4; i += blockDim.x) lds[i] = p.job.sc.ops[i];
-    for (int i = tid; i < nP; i += blockDim.x) {
-      lds[nOps4 + i] = p.job.sc.primA[i];
-      lds[nOps4 + nP + i] = p.job.sc.primB[i];
-    }
-    sv0 = SceneView{lds, lds + nOps4, lds + nOps4 + nP};
-    sceneBytes = static_cast<size_t>(nOps4 + 2 * nP) * sizeof(float4);
-  }
-  const size_t perWave = static_cast<size_t>(p.coopOps) * sizeof(OpState) + static_cast<size_t>(p.coopQueueCap) * 4;
-  OpState* st = reinterpret_cast<OpState*>(reinterpret_cast<char*>(ldsBase) + sceneBytes + wave * perWave);
+  const int lane = threadIdx.x;
+  if (lane < 27) table[lane] = kActionTable[lane];
+  OpState* st = reinterpret_cast<OpState*>(ldsBase);
   uint32_t* q = reinterpret_cast<uint32_t*>(st + p.coopOps);
-  uint32_t* shared = sharedAll + 4 * wave;
   const uint32_t qmask = p.coopQueueCap - 1;
-  __syncthreads();  // the only workgroup barrier: the waves part here
+  __syncthreads();
   const uint32_t listed = __hip_atomic_load(p.queue + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   const uint32_t count = listed < p.heavyCap ? listed : p.heavyCap;
   // The loop body must not end in a branch on the lane index: the compiler
@@ -970,7 +978,7 @@ __global__ void __launch_bounds__(1024) CoopKernel(RenderParams p) {
     if (k >= count) break;
     const uint2 e = p.heavy[k];
     const Job& jb = kBatch ? p.jobs[__builtin_amdgcn_readfirstlane(e.x)] : p.job;
-    const SceneView sv = kBatch ? SceneView{jb.sc.ops, jb.sc.primA, jb.sc.primB} : sv0;
+    const SceneView sv{jb.sc.ops, jb.sc.primA, jb.sc.primB};
     const uint32_t out = __builtin_amdgcn_readfirstlane(e.y);
     const uint32_t spp = static_cast<uint32_t>(jb.spp);
     const uint32_t pixel = out / spp, sample = out - pixel * spp;
@@ -983,7 +991,7 @@ __global__ void __launch_bounds__(1024) CoopKernel(RenderParams p) {
     }
     const RayF r = MakeRay(jb, x, y, ox, oy);
     Counts n{0, 0, 0, 0, 0, 0};
-    const Hit h = CoopTrace<kFast, kCount>(sv, r, st, q, qmask, table, shared, lane, n);
+    const Hit h = CoopTrace<kFast, kCount>(sv, r, st, q, qmask, table, lane, n);
     if (kCount && jb.counters) {
       int32_t v[6] = {n.go, n.comp, n.box, n.sph, n.cyl, n.cub};
       for (int c = 0; c < 6; c++) {
@@ -1024,18 +1032,14 @@ hipError_t Launch(const RenderParams& p, const LaunchShape& s, hipStream_t strea
   hipError_t e = hipGetLastError();
   if (e != hipSuccess || p.budget <= 0) return e;
   // the heavy rays the first kernel handed over
-  auto coop = CoopKernel<kFast, kCount, kBatch, false>;
-  if constexpr (!kBatch) {
-    if (p.coopLds) coop = CoopKernel<kFast, kCount, false, true>;
-  }
+  auto coop = CoopKernel<kFast, kCount, kBatch>;
   const size_t lds = CoopLaunchLds(p);
   if (lds > 64 * 1024) {
     e = hipFuncSetAttribute(reinterpret_cast<const void*>(coop), hipFuncAttributeMaxDynamicSharedMemorySize,
                             static_cast<int>(lds));
     if (e != hipSuccess) return e;
   }
-  const uint32_t wavesPerGroup = static_cast<uint32_t>(p.coopBlock / 64);
-  hipLaunchKernelGGL(coop, dim3(p.coopWaves / wavesPerGroup), dim3(p.coopBlock), lds, stream, p);
+  hipLaunchKernelGGL(coop, dim3(p.coopWaves), dim3(64), lds, stream, p);
   return hipGetLastError();
 }
 
@@ -1069,15 +1073,7 @@ const DeviceLimits* CurrentLimits() {
 
 int32_t MaxSupportedDepth() { return kMaxDepth; }
 
-size_t CoopWaveLds(int32_t ops, uint32_t queueCap) {
-  return static_cast<size_t>(ops) * sizeof(OpState) + static_cast<size_t>(queueCap) * 4;
-}
-
-size_t CoopLaunchLds(const RenderParams& p) {
-  const size_t scene =
-      p.coopLds ? (static_cast<size_t>(p.job.sc.nOps) * 4 + static_cast<size_t>(p.job.sc.nPrims) * 2) * 16 : 0;
-  return scene + static_cast<size_t>(p.coopBlock / 64) * CoopWaveLds(p.coopOps, p.coopQueueCap);
-}
+size_t CoopLaunchLds(const RenderParams& p) { return CoopWaveLds(p.coopOps, p.coopQueueCap); }
 
 size_t CoopLdsBytes(int32_t ops, uint32_t* queueCap) {
   uint32_t cap = 1;

@@ -249,10 +249,10 @@ typedef struct csg_launch_info {
   int32_t compute_units;
   int64_t lds_limit;      /* LDS per workgroup the device allows */
   int32_t coop_budget;    /* steps before a ray is handed to the cooperative kernel (0: never) */
-  int32_t coop_grid;      /* workgroups of the cooperative kernel */
-  int32_t coop_block;     /* its threads per workgroup (64 per ray in flight) */
+  int32_t coop_grid;      /* workgroups of the cooperative kernel (one ray in flight each) */
+  int32_t coop_block;     /* its threads per workgroup (64) */
   int32_t reserved0;
-  int64_t coop_lds_bytes; /* its dynamic LDS per workgroup (the scene copy included when staged) */
+  int64_t coop_lds_bytes; /* its dynamic LDS per workgroup */
 } csg_launch_info;
 csg_status csg_render_plan(const csg_scene* scene, const csg_render_opts* opts, csg_launch_info* info);
 

@@ -2,6 +2,7 @@
 function include/csg.h declares, and its host-side pieces (camera model,
 light and shading defaults, scene info, argument checks) match the
 reference's definitions and the oracle."""
+import ctypes
 import math
 import os
 import re
@@ -126,3 +127,15 @@ def test_jitter_hash_matches_oracle_and_is_uniform():
         vals.append(a)
     hist = np.histogram(vals, bins=10, range=(0, 1))[0]
     assert hist.min() > 140 and hist.max() < 260
+
+
+@pytest.mark.parametrize("field,value", [("coop_when", 2), ("coop_when", -1), ("coop_grace", 100001),
+                                         ("samples", -1), ("samples", 1025), ("mode", 7)])
+def test_bad_render_options_are_rejected_before_any_device_call(field, value):
+    s = C.Scene(scene_text("test1.txt"), upload=False)
+    opts = C.RenderOpts(C.MODE_FAST, 0, C.PLACE_AUTO, 0, None, None, 1, 0, 0, 0, 0, 0)
+    setattr(opts, field, value)
+    info = C.LaunchInfo()
+    st = C.lib().csg_render_plan(s.handle, ctypes.byref(opts), ctypes.byref(info))
+    assert st == C.E_ARG, (field, value, st)
+    s.close()
