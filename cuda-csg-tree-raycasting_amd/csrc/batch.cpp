@@ -114,8 +114,9 @@ csg_status Stage(csg_batch* b) {
     p.queue = ch.queue;
     p.morton = b->cfg.morton ? 1 : 0;
     if (cnt == 0) std::memset(&ch.hostJobs[0], 0, sizeof(csgray::Job));
-    const hipError_t e = csgray::ChooseShape(depth, 0, b->cfg, &ch.shape);
+    const hipError_t e = csgray::ChooseShape(depth, 0, b->cfg, &ch.shape, std::max(cnt, 1));
     if (e != hipSuccess) return FailHip(e, "choosing the launch shape");
+    csgray::FitGridToTiles(&ch.shape, base);
     p.totalWaves = static_cast<uint32_t>(ch.shape.grid) * static_cast<uint32_t>(ch.shape.block / 64);
     PrepareCoop(b->cfg, maxOps, ch.heavy, &p);
   }

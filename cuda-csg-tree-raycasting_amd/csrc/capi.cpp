@@ -554,6 +554,7 @@ csg_status csg_render_ex(const csg_scene* s, const csg_camera* cam, const csg_li
   csgray::LaunchShape shape;
   e = csgray::ChooseShape(s->dev.opDepth, SceneBytes(s), cfg, &shape);
   if (e != hipSuccess) return FailHip(e, "choosing the launch shape");
+  csgray::FitGridToTiles(&shape, p.numTiles);
   p.totalWaves = static_cast<uint32_t>(shape.grid) * static_cast<uint32_t>(shape.block / 64);
   const hipStream_t hs = static_cast<hipStream_t>(stream);
   const csg_status qs = QueueFor(dev, hs, &p.queue);
@@ -621,8 +622,9 @@ csg_status csg_render_batch(const csg_job* jobs, int32_t njobs, const csg_render
     p.stackSlots = depth;
     p.morton = cfg.morton ? 1 : 0;
     csgray::LaunchShape shape;
-    e = csgray::ChooseShape(depth, 0, cfg, &shape);
+    e = csgray::ChooseShape(depth, 0, cfg, &shape, n);
     if (e != hipSuccess) return FailHip(e, "choosing the launch shape");
+    csgray::FitGridToTiles(&shape, p.numTiles);
     p.totalWaves = static_cast<uint32_t>(shape.grid) * static_cast<uint32_t>(shape.block / 64);
     const csg_status qs = QueueFor(dev, hs, &p.queue);
     if (qs != CSG_OK) return qs;

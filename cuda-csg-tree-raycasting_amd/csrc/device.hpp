@@ -107,10 +107,19 @@ struct LaunchShape {
   bool sceneInLds = false;
 };
 
+// Fewer workgroups when the launch has fewer tiles than the grid has waves:
+// the extra waves would only pop an empty queue (and stage the scene).
+inline void FitGridToTiles(LaunchShape* s, int64_t tiles) {
+  const int64_t waves = s->block / 64;
+  const int64_t need = (tiles + waves - 1) / waves;
+  if (need < s->grid) s->grid = static_cast<int>(need > 1 ? need : 1);
+}
+
 // Picks the launch shape on the current device for a launch whose scenes
 // have at most `depth` operator levels; `sceneBytes` > 0 allows staging that
 // many bytes of scene in LDS (single-scene launches only).
-hipError_t ChooseShape(int32_t depth, size_t sceneBytes, const LaunchConfig& cfg, LaunchShape* shape);
+hipError_t ChooseShape(int32_t depth, size_t sceneBytes, const LaunchConfig& cfg, LaunchShape* shape,
+                       int32_t batchJobs = 0);
 
 // Compute units and LDS bytes per workgroup of the current device.
 hipError_t DeviceShapeLimits(int* computeUnits, size_t* ldsPerBlock);

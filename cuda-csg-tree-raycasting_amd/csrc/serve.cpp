@@ -163,7 +163,9 @@ void Dispatch(csg_server* s, Slot& sl) {
   csgray::RenderParams p{};
   csg_status st = UploadJobs(s->device, sl.stream, jobs.data(), jobs.size(), &p.jobs);
   csgray::LaunchShape shape;
-  if (st == CSG_OK && csgray::ChooseShape(depth, 0, s->launch, &shape) != hipSuccess) st = CSG_E_HIP;
+  if (st == CSG_OK &&
+      csgray::ChooseShape(depth, 0, s->launch, &shape, static_cast<int32_t>(jobs.size())) != hipSuccess)
+    st = CSG_E_HIP;
   uint32_t* queue = nullptr;
   if (st == CSG_OK) st = QueueFor(s->device, sl.stream, &queue);
   uint2* heavy = nullptr;
@@ -173,6 +175,7 @@ void Dispatch(csg_server* s, Slot& sl) {
     p.numTiles = static_cast<int32_t>(base);
     p.stackSlots = depth;
     p.queue = queue;
+    csgray::FitGridToTiles(&shape, base);
     p.totalWaves = static_cast<uint32_t>(shape.grid) * static_cast<uint32_t>(shape.block / 64);
     int32_t maxOps = 1;
     for (int64_t tk : take) maxOps = std::max(maxOps, s->scenes[s->reqs[tk].req.scene_id]->dev.nOps);
