@@ -42,9 +42,21 @@ def flops_per_ray(ev):
             + FLOPS_CUBE * ev["cube"] + FLOPS_COMPUTE * ev["compute"] + FLOPS_HIT * ev["hit"])
 
 
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(text, cam6, light3, phong8, w, h, row_step):
-    """The oracle's FP32 restatement (the reference's arithmetic) on all host
-    cores over every `row_step`-th image row."""
+    """The oracle's FP32 restatement (the reference's arithmetic) on up to 16
+    host cores over every `row_step`-th image row, plus one core over every
+    16th row."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     o = O.OracleScene(text)
@@ -53,9 +65,15 @@ def cpu_baseline(text, cam6, light3, phong8, w, h, row_step):
     t0 = time.perf_counter()
     o.render(cam6, light3, phong8, w, h, bits=32, threads=threads, rows=(0, h, row_step))
     dt = time.perf_counter() - t0
+    rows1 = len(range(0, h, 16))
+    t1 = time.perf_counter()
+    o.render(cam6, light3, phong8, w, h, bits=32, threads=1, rows=(0, h, 16))
+    dt1 = time.perf_counter() - t1
     return {"value": round(rows * w / dt / 1e6, 4), "unit": "Mrays/s", "cores": threads, "kind": "port",
+            "single_core_mrays_s": round(rows1 * w / dt1 / 1e6, 4), "cpu_model": _cpu_model(),
+            "host_cpus": os.cpu_count(),
             "sample": f"every {row_step}th row of the {w}x{h} frame ({rows * w} rays), FP32 oracle, "
-                      f"{threads} threads, {dt:.1f} s"}
+                      f"{threads} threads, {dt:.1f} s; single core: every 16th row ({rows1 * w} rays), {dt1:.1f} s"}
 
 
 def _time_ms(torch, fn, reps):
