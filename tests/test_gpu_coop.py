@@ -190,3 +190,21 @@ def test_plan_reports_cooperative_kernel(gpu):
     assert s.plan(coop=-1).coop_budget == 0
     assert s.plan(coop=7).coop_budget == 7
     s.close()
+
+
+def test_default_tail_hand_over_engages_on_the_benchmark_frame(gpu):
+    """With default options the cheese frame at 1920x1080 hands its tail to
+    the cooperative kernel (tens of thousands of rays), while turning the
+    hand-over off moves none; the images are identical either way."""
+    print("\n[case] cheese 512 1920x1080 fast, default hand-over vs off", flush=True)
+    sname, cam, light, phong, cam6, l3, ph8 = setups()[1]
+    s = C.Scene(S.cheese(512, seed=512))
+    before = C.coop_rays()
+    a = _np(s.render(cam, light, phong, 1920, 1080))
+    moved = C.coop_rays() - before
+    b = _np(s.render(cam, light, phong, 1920, 1080, coop=-1))
+    off = C.coop_rays() - before - moved
+    s.close()
+    print(f"rays handed over: {moved} (off: {off})")
+    assert moved > 1000 and off == 0
+    _same(a, b)

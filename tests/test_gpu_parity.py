@@ -214,3 +214,26 @@ def test_render_argument_errors(gpu):
     out = host.render(cam, light, phong, 16, 16)
     torch.cuda.synchronize()
     assert out["prim_id"].shape == (16, 16)
+
+
+@pytest.mark.parametrize("seed", range(30))
+def test_random_tree_exact_matches_fp32_oracle(gpu, seed):
+    """Random trees of the batched mix (2-1024 leaves, all three primitive
+    kinds, random operators, balanced) from random orbit cameras, rendered
+    alone in exact mode: bit-identical to the FP32 oracle, counters included
+    (rays of heavy trees also pass through the cooperative kernel)."""
+    from csgray import scenes as S
+    rng = np.random.default_rng(1000 + seed)
+    text = S.random_tree(rng)
+    eye = S.orbit_eye(rng, 4.0, 80.0)
+    _label(f"random tree seed {seed}", "orbit")
+    cam = C.camera_look_at(eye, (0, 0, 0), 60)
+    light, phong = C.light(S.LIGHT_TOWARD), C.phong(*S.PHONG)
+    w, h = 96, 72
+    g = gpu_render(text, cam, light, phong, w, h, C.MODE_EXACT, counters=True)
+    o = O.OracleScene(text).render(C.cam_tuple(cam), S.LIGHT_TOWARD, O.phong8(*S.PHONG), w, h, bits=32,
+                                   counters=True)
+    assert np.array_equal(g["prim_id"], o["prim_id"])
+    assert np.array_equal(g["flags"], o["flags"])
+    assert np.array_equal(g["depth"].view(np.uint32), o["depth"].view(np.uint32))
+    assert np.array_equal(g["counters"], o["counters"])
