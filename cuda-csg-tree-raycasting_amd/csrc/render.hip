@@ -56,7 +56,7 @@ constexpr uint32_t kNone = 0xFFFFFFFFu;
 
 // Action tables (RaycastingKernels.cu:666-677), index op*9 + l*3 + r with
 // states Enter 0, Exit 1, Miss 2 and ops Union 0, Difference 1, Intersection 2.
-__constant__ uint16_t kActionTable[27] = {
+constexpr uint16_t kActionTable[27] = {
     // Union
     kRetLIfCloser | kRetRIfCloser, kRetRIfCloser | kLoopL, kRetL,
     kRetLIfCloser | kLoopR, kLoopLIfCloser | kLoopRIfCloser, kRetL,
@@ -70,6 +70,46 @@ __constant__ uint16_t kActionTable[27] = {
     kRetRIfCloser | kLoopL, kRetLIfCloser | kRetRIfCloser, kAMiss,
     kAMiss, kAMiss, kAMiss,
 };
+
+// Outcome of a Compute (:597-661): which record is returned or which child
+// is looped into.
+constexpr uint32_t kOutMiss = 0, kOutRetL = 1, kOutRetR = 2, kOutRetRFlip = 3, kOutLoopL = 4, kOutLoopR = 5;
+
+// Compute's decision taken in the reference's order of precedence (return
+// left, return right, loop left, loop right, miss) for one action mask and
+// one comparison of the two records' distances.
+constexpr uint32_t DecisionOutcome(uint32_t acts, bool lCloser, bool rCloser) {
+  const bool retL = (acts & kRetL) || ((acts & kRetLIfCloser) && lCloser);
+  const bool retR = !retL && ((acts & kRetR) || ((acts & kRetRIfCloser) && rCloser));
+  const bool loopL = !retL && !retR && ((acts & kLoopL) || ((acts & kLoopLIfCloser) && lCloser));
+  const bool loopR = !retL && !retR && !loopL && ((acts & kLoopR) || ((acts & kLoopRIfCloser) && rCloser));
+  return retL ? kOutRetL : retR ? ((acts & kFlipR) ? kOutRetRFlip : kOutRetR) : loopL ? kOutLoopL
+                                                                                   : loopR ? kOutLoopR : kOutMiss;
+}
+
+// The action table folded with the state lookup (LookUpActions, :679-691)
+// and the distance comparison: entry op*16 + l*4 + r, where l and r are a
+// record's Enter and Exit flag bits ((bits >> 1) & 3: Enter wins over
+// Exit, neither is Miss), holds the outcome for "left closer" in bits 0-2,
+// "right closer" in bits 3-5 and "neither" (equal or NaN) in bits 6-8.
+struct DecisionTable {
+  uint16_t v[48];
+};
+constexpr uint32_t StateOfFlagBits(uint32_t code) { return (code & 1u) ? 0u : ((code & 2u) ? 1u : 2u); }
+constexpr DecisionTable MakeDecisionTable() {
+  DecisionTable t{};
+  for (uint32_t op = 0; op < 3; op++)
+    for (uint32_t l = 0; l < 4; l++)
+      for (uint32_t r = 0; r < 4; r++) {
+        const uint32_t acts = kActionTable[op * 9 + StateOfFlagBits(l) * 3 + StateOfFlagBits(r)];
+        t.v[op * 16 + l * 4 + r] = static_cast<uint16_t>(DecisionOutcome(acts, true, false) |
+                                                         (DecisionOutcome(acts, false, true) << 3) |
+                                                         (DecisionOutcome(acts, false, false) << 6));
+      }
+  return t;
+}
+__constant__ DecisionTable kDecisions = MakeDecisionTable();
+constexpr int kDecisionEntries = 48;
 
 // A single-hit record: the reference's RayHitMinimal (Ray.cuh:36-52) with
 // hit flags in bits 0-7, primitive type in bits 8-10, and the int16
@@ -96,8 +136,12 @@ constexpr uint32_t kSlotLoadLft = 1u << 11, kSlotLoadRgh = 2u << 11, kSlotSaveLf
 __device__ __forceinline__ Hit MissRecord() { return Hit{-1.0f, kMissBit | 0xFFFF0000u}; }
 __device__ __forceinline__ uint32_t Flags(uint32_t bits) { return bits & 0xFFu; }
 __device__ __forceinline__ uint32_t WithFlags(uint32_t bits, uint32_t f) { return (bits & ~0xFFu) | f; }
-__device__ __forceinline__ uint32_t State(uint32_t bits) {  // LookUpActions, :679-691
-  return (bits & kEnter) ? 0u : ((bits & kExit) ? 1u : 2u);
+// Compute's outcome for operator type `op` (0-2) and records L, R, read from
+// the decision table `dtab` (an LDS copy of kDecisions).
+__device__ __forceinline__ uint32_t Decide(const uint16_t* dtab, uint32_t op, const Hit& L, const Hit& R) {
+  const uint32_t e = dtab[op * 16u + ((L.bits & (kEnter | kExit)) << 1) + ((R.bits & (kEnter | kExit)) >> 1)];
+  const uint32_t shift = L.t < R.t ? 0u : (L.t > R.t ? 3u : 6u);
+  return (e >> shift) & 7u;
 }
 __device__ __forceinline__ uint32_t HitBits(uint32_t flags, uint32_t type, uint32_t prim) {
   return flags | (type << 8) | (prim << 16);
@@ -149,10 +193,10 @@ __device__ __forceinline__ bool BoxHit(const RayF& r, float lx, float ly, float 
                                        float tmin) {
   float tmn, tmx;
   Slab<kFast>(r, lx, ly, lz, hx, hy, hz, tmn, tmx);
-  if (tmx < 0) return false;
-  if (tmn > tmx) return false;
-  if (tmn <= tmin && tmx <= tmin) return false;
-  return true;
+  // The reference's three early returns as one expression (equal for every
+  // input, NaN included): evaluated without branches, the slab's min/max
+  // stay in one block with the products that feed them.
+  return !(tmx < 0) & !(tmn > tmx) & !((tmn <= tmin) & (tmx <= tmin));
 }
 
 // sphereHit (:135-181).
@@ -463,22 +507,15 @@ __device__ Hit Trace(const SceneView& sv, const RayF& r, uint32_t rootDesc, uint
       if (kCount) n.comp++;
       L = action == kLoadLft ? held : L;
       R = action == kLoadRgh ? held : R;
-      // Speculative reads, all issued before the decision: the action, the
+      // Speculative reads, all issued before the decision: the decision, the
       // stack top (popped if the operator returns) and the parent's
       // descriptor word (the root reads operator 0's and ignores it).
-      const uint32_t acts = table[opType * 9u + State(L.bits) * 3u + State(R.bits)];
+      const uint32_t dec = Decide(table, opType, L, R);
       const int top = min(max(sp - 1, 0), cap - 1);
       const uint2 e = stk[top * stride];
       const float4 pq = sv.ops[4 * (opParent == kNone ? 0u : opParent) + 3];
-      const bool lc = L.t < R.t, rc = L.t > R.t;
-      const bool aRetL = (acts & kRetL) != 0, aRetLc = (acts & kRetLIfCloser) != 0;
-      const bool aRetR = (acts & kRetR) != 0, aRetRc = (acts & kRetRIfCloser) != 0;
-      const bool aLoopL = (acts & kLoopL) != 0, aLoopLc = (acts & kLoopLIfCloser) != 0;
-      const bool aLoopR = (acts & kLoopR) != 0, aLoopRc = (acts & kLoopRIfCloser) != 0;
-      const bool retL = aRetL | (aRetLc & lc);
-      const bool retR = !retL & (aRetR | (aRetRc & rc));
-      const bool loopL = !retL & !retR & (aLoopL | (aLoopLc & lc));
-      const bool loopR = !retL & !retR & !loopL & (aLoopR | (aLoopRc & rc));
+      const bool retL = dec == kOutRetL, retR = (dec & ~1u) == kOutRetR;
+      const bool loopL = dec == kOutLoopL, loopR = dec == kOutLoopR;
       if (loopL | loopR) {
         tmin = loopL ? L.t : R.t;
         const Hit park = loopL ? R : L;
@@ -488,7 +525,7 @@ __device__ Hit Trace(const SceneView& sv, const RayF& r, uint32_t rootDesc, uint
         sp++;
         action = loopL ? kGotoLft : kGotoRgh;
       } else {
-        const uint32_t rb = (acts & kFlipR) ? (R.bits ^ (kFlip | kExit | kEnter)) : R.bits;
+        const uint32_t rb = dec == kOutRetRFlip ? (R.bits ^ (kFlip | kExit | kEnter)) : R.bits;
         const Hit res = retL ? L : (retR ? Hit{R.t, rb} : MissRecord());
         L = res;
         R = res;
@@ -717,13 +754,13 @@ __device__ __forceinline__ void RenderTile(const Job& jb, const SceneView& sv, u
 template <bool kFast, bool kCount, bool kLds, bool kBatch>
 __device__ __forceinline__ void RenderBody(const RenderParams& p) {
   extern __shared__ uint4 ldsBase[];
-  __shared__ uint16_t table[27];
+  __shared__ uint16_t table[kDecisionEntries];
   __shared__ uint64_t ldsStamps[2];
   const int tid = threadIdx.x;
   const int cap = p.stackSlots > 0 ? p.stackSlots : 1;
   if (tid < 2) ldsStamps[tid] = 0;
   uint2* slab = reinterpret_cast<uint2*>(ldsBase);
-  if (tid < 27) table[tid] = kActionTable[tid];
+  if (tid < kDecisionEntries) table[tid] = kDecisions.v[tid];
   int32_t* jobBase = reinterpret_cast<int32_t*>(slab + static_cast<size_t>(cap) * blockDim.x);
 
   SceneView sv{p.job.sc.ops, p.job.sc.primA, p.job.sc.primB};
@@ -941,16 +978,9 @@ __device__ Hit CoopTrace(const SceneView& sv, const RayF& r, OpState* st, uint32
         if (kCount) n.comp++;
         const Hit L{st[X].lt, st[X].lb}, R{st[X].rt, st[X].rb};
         const uint32_t typeWord = AsU(q3.z), parent = AsU(q3.w);
-        const uint32_t acts = table[(typeWord & 0xFFu) * 9u + State(L.bits) * 3u + State(R.bits)];
-        const bool lc = L.t < R.t, rc = L.t > R.t;
-        const bool aRetL = (acts & kRetL) != 0, aRetLc = (acts & kRetLIfCloser) != 0;
-        const bool aRetR = (acts & kRetR) != 0, aRetRc = (acts & kRetRIfCloser) != 0;
-        const bool aLoopL = (acts & kLoopL) != 0, aLoopLc = (acts & kLoopLIfCloser) != 0;
-        const bool aLoopR = (acts & kLoopR) != 0, aLoopRc = (acts & kLoopRIfCloser) != 0;
-        const bool retL = aRetL | (aRetLc & lc);
-        const bool retR = !retL & (aRetR | (aRetRc & rc));
-        const bool loopL = !retL & !retR & (aLoopL | (aLoopLc & lc));
-        const bool loopR = !retL & !retR & !loopL & (aLoopR | (aLoopRc & rc));
+        const uint32_t dec = Decide(table, typeWord & 0xFFu, L, R);
+        const bool retL = dec == kOutRetL, retR = (dec & ~1u) == kOutRetR;
+        const bool loopL = dec == kOutLoopL, loopR = dec == kOutLoopR;
         if (loopL | loopR) {
           const float tmin = loopL ? L.t : R.t;
           const uint32_t d = loopL ? lDesc : rDesc;
@@ -972,7 +1002,7 @@ __device__ Hit CoopTrace(const SceneView& sv, const RayF& r, OpState* st, uint32
             sp0 = EnterTask(DescIndex(d));
           }
         } else {
-          const uint32_t rb = (acts & kFlipR) ? (R.bits ^ (kFlip | kExit | kEnter)) : R.bits;
+          const uint32_t rb = dec == kOutRetRFlip ? (R.bits ^ (kFlip | kExit | kEnter)) : R.bits;
           out = retL ? L : (retR ? Hit{R.t, rb} : MissRecord());
           if (parent == kNone) {
             fin = true;
@@ -1014,9 +1044,9 @@ __device__ Hit CoopTrace(const SceneView& sv, const RayF& r, OpState* st, uint32
 template <bool kFast, bool kCount, bool kBatch>
 __global__ void __launch_bounds__(64) CoopKernel(RenderParams p) {
   extern __shared__ uint4 ldsBase[];
-  __shared__ uint16_t table[27];
+  __shared__ uint16_t table[kDecisionEntries];
   const int lane = threadIdx.x;
-  if (lane < 27) table[lane] = kActionTable[lane];
+  if (lane < kDecisionEntries) table[lane] = kDecisions.v[lane];
   OpState* st = reinterpret_cast<OpState*>(ldsBase);
   uint32_t* q = reinterpret_cast<uint32_t*>(st + p.coopOps);
   const uint32_t qmask = p.coopQueueCap - 1;
