@@ -434,7 +434,10 @@ __device__ Hit Trace(const SceneView& sv, const RayF& r, uint32_t rootDesc, uint
   // At the top of the loop action is GotoLft, GotoRgh, Compute, LoadLft or
   // LoadRgh.
   int action = kGotoLft;
+  // Stack pointer as an element offset (slots x stride): pushes and pops
+  // add and subtract `stride`, no multiplies.
   int sp = 0;
+  const int capE = cap * stride, lastE = capE - stride;
   bool done = false, aborted = false;
   int iter = 0;
 
@@ -451,8 +454,8 @@ __device__ Hit Trace(const SceneView& sv, const RayF& r, uint32_t rootDesc, uint
         L = left ? h : L;
         R = left ? R : h;
         aborted |= sp <= 0;
-        sp = max(sp - 1, 0);
-        const uint2 e = stk[sp * stride];
+        sp = max(sp - stride, 0);
+        const uint2 e = stk[sp];
         held = Hit{__uint_as_float(e.x), e.y & ~kSlotMask};
         action = (e.y & kSlotMask) == kSlotLoadLft ? kLoadLft : kLoadRgh;
       } else {
@@ -489,9 +492,9 @@ __device__ Hit Trace(const SceneView& sv, const RayF& r, uint32_t rootDesc, uint
         const uint32_t pb = both ? kSlotSaveLft
                                  : (dR ? ((L.bits & ~kSlotMask) | kSlotLoadLft) : ((R.bits & ~kSlotMask) | kSlotLoadRgh));
         if (dL | dR) {
-          aborted |= sp >= cap;
-          stk[min(sp, cap - 1) * stride] = make_uint2(AsU(pt), pb);
-          sp++;
+          aborted |= sp >= capE;
+          stk[min(sp, lastE)] = make_uint2(AsU(pt), pb);
+          sp += stride;
         }
         action = dL ? kGotoLft : (dR ? kGotoRgh : kCompute);
       }
@@ -511,18 +514,22 @@ __device__ Hit Trace(const SceneView& sv, const RayF& r, uint32_t rootDesc, uint
       // stack top (popped if the operator returns) and the parent's
       // descriptor word (the root reads operator 0's and ignores it).
       const uint32_t dec = Decide(table, opType, L, R);
-      const int top = min(max(sp - 1, 0), cap - 1);
-      const uint2 e = stk[top * stride];
+      const int top = min(max(sp - stride, 0), lastE);
+      const uint2 e = stk[top];
       const float4 pq = sv.ops[4 * (opParent == kNone ? 0u : opParent) + 3];
+      // Keep the three reads ahead of the branch below (one LDS round trip):
+      // left alone, the compiler sinks the stack and parent reads into the
+      // return path, after the decision's read has completed.
+      asm volatile("" ::"v"(dec), "v"(e.x), "v"(e.y), "v"(pq.x), "v"(pq.y), "v"(pq.z), "v"(pq.w));
       const bool retL = dec == kOutRetL, retR = (dec & ~1u) == kOutRetR;
       const bool loopL = dec == kOutLoopL, loopR = dec == kOutLoopR;
       if (loopL | loopR) {
         tmin = loopL ? L.t : R.t;
         const Hit park = loopL ? R : L;
-        aborted |= sp >= cap;
-        stk[min(sp, cap - 1) * stride] =
+        aborted |= sp >= capE;
+        stk[min(sp, lastE)] =
             make_uint2(AsU(park.t), (park.bits & ~kSlotMask) | (loopL ? kSlotLoadRgh : kSlotLoadLft));
-        sp++;
+        sp += stride;
         action = loopL ? kGotoLft : kGotoRgh;
       } else {
         const uint32_t rb = dec == kOutRetRFlip ? (R.bits ^ (kFlip | kExit | kEnter)) : R.bits;
@@ -542,8 +549,8 @@ __device__ Hit Trace(const SceneView& sv, const RayF& r, uint32_t rootDesc, uint
           // SaveLft: tmin parked by the GoTo that went left comes back and
           // the left result takes its slot; then go right.
           tmin = held.t;
-          stk[sp * stride] = make_uint2(AsU(L.t), (L.bits & ~kSlotMask) | kSlotLoadLft);
-          sp++;
+          stk[sp] = make_uint2(AsU(L.t), (L.bits & ~kSlotMask) | kSlotLoadLft);
+          sp += stride;
           action = kGotoRgh;
         } else {
           action = code == kSlotLoadLft ? kLoadLft : kLoadRgh;
