@@ -153,7 +153,8 @@ __device__ __forceinline__ uint32_t DescIndex(uint32_t d) { return d >> 8; }
 struct RayF {
   float ox, oy, oz;
   float dx, dy, dz;
-  float ix, iy, iz;  // 1/d (fast mode)
+  float ix, iy, iz;     // 1/d (fast mode)
+  float nox, noy, noz;  // -o/d (fast mode bounding-box slabs)
 };
 
 // Saturating truncation of the reference's (float)(int)q (:424, :449): NaN
@@ -188,18 +189,34 @@ __device__ __forceinline__ void Slab(const RayF& r, float lx, float ly, float lz
 }
 
 // isBVHNodeHit (:718-757).  The caller turns a miss into hit = Miss.
+// Fast mode computes each slab distance as one fma, l * (1/d) + (-o/d).
+// Where a direction component is exactly zero that gives NaN in place of
+// the reference's +-inf, and fminf/fmaxf then ignore the axis: the test can
+// only pass more rays than the reference's, which costs a visit and never
+// changes a result (a box only culls; the cube primitive, whose slab is its
+// answer, keeps the reference's form).
 template <bool kFast>
 __device__ __forceinline__ bool BoxHit(const RayF& r, float lx, float ly, float lz, float hx, float hy, float hz,
                                        float tmin) {
   float tmn, tmx;
-  Slab<kFast>(r, lx, ly, lz, hx, hy, hz, tmn, tmx);
+  if (kFast) {
+    const float t1 = fmaf(lx, r.ix, r.nox), t2 = fmaf(hx, r.ix, r.nox);
+    const float t3 = fmaf(ly, r.iy, r.noy), t4 = fmaf(hy, r.iy, r.noy);
+    const float t5 = fmaf(lz, r.iz, r.noz), t6 = fmaf(hz, r.iz, r.noz);
+    tmn = fmaxf(fmaxf(fminf(t1, t2), fminf(t3, t4)), fminf(t5, t6));
+    tmx = fminf(fminf(fmaxf(t1, t2), fmaxf(t3, t4)), fmaxf(t5, t6));
+  } else {
+    Slab<false>(r, lx, ly, lz, hx, hy, hz, tmn, tmx);
+  }
   // The reference's three early returns as one expression (equal for every
   // input, NaN included): evaluated without branches, the slab's min/max
   // stay in one block with the products that feed them.
   return !(tmx < 0) & !(tmn > tmx) & !((tmn <= tmin) & (tmx <= tmin));
 }
 
-// sphereHit (:135-181).
+// sphereHit (:135-181).  Fast mode takes the hardware square root (about
+// 1 ulp) in place of the correctly rounded one.
+template <bool kFast>
 __device__ __forceinline__ Hit SphereHit(const RayF& r, const float4& a, uint32_t prim, float tmin, Hit h) {
   h.bits = HitBits(kMissBit, 0, prim);
   const float ocx = r.ox - a.x, ocy = r.oy - a.y, ocz = r.oz - a.z;
@@ -207,7 +224,7 @@ __device__ __forceinline__ Hit SphereHit(const RayF& r, const float4& a, uint32_
   const float c = (ocx * ocx + ocy * ocy + ocz * ocz) - a.w * a.w;
   const float disc = b * b - c;
   if (disc < 0) return h;
-  const float sq = sqrtf(disc);
+  const float sq = kFast ? __builtin_amdgcn_sqrtf(disc) : sqrtf(disc);
   float temp = (-b - sq);
   if (temp <= tmin) {
     temp = (-b + sq);
@@ -375,7 +392,7 @@ __device__ __forceinline__ Hit Intersect(const SceneView& sv, const RayF& r, uin
   const float4 a = sv.primA[prim];
   if (type == kSphereT) {
     if (kCount) n.sph++;
-    return SphereHit(r, a, prim, tmin, h);
+    return SphereHit<kFast>(r, a, prim, tmin, h);
   }
   if (type == kCylinderT) {
     if (kCount) n.cyl++;
@@ -613,6 +630,9 @@ __device__ __forceinline__ RayF MakeRay(const Job& jb, int x, int y, float ox, f
   r.ix = 1.0f / r.dx;
   r.iy = 1.0f / r.dy;
   r.iz = 1.0f / r.dz;
+  r.nox = -r.ox * r.ix;
+  r.noy = -r.oy * r.iy;
+  r.noz = -r.oz * r.iz;
   return r;
 }
 
@@ -937,14 +957,18 @@ __device__ Hit CoopTrace(const SceneView& sv, const RayF& r, OpState* st, uint32
     Hit out = MissRecord();
     if (mine != kNone) {
       const uint32_t X = mine >> 1;
+      // Everything either kind of task reads about operator X, issued at
+      // once (one L2 round trip for the record, one LDS trip for the state).
+      const float4 q0 = sv.ops[4 * X + 0];
+      const float4 q1 = sv.ops[4 * X + 1];
+      const float4 q2 = sv.ops[4 * X + 2];
       const float4 q3 = sv.ops[4 * X + 3];
+      const OpState sx = st[X];
+      asm volatile("" ::"v"(q0.x), "v"(q1.x), "v"(q2.x), "v"(q3.x), "v"(sx.tmin), "v"(sx.lt));
       const uint32_t lDesc = AsU(q3.x), rDesc = AsU(q3.y);
       if ((mine & 1u) == 0) {
         if (kCount) n.go++;
-        const float tmin = st[X].tmin;
-        const float4 q0 = sv.ops[4 * X + 0];
-        const float4 q1 = sv.ops[4 * X + 1];
-        const float4 q2 = sv.ops[4 * X + 2];
+        const float tmin = sx.tmin;
         const bool gL = BoxHit<kFast>(r, q0.x, q0.y, q0.z, q1.x, q1.y, q1.z, tmin);
         const bool gR = BoxHit<kFast>(r, q0.w, q1.w, q2.x, q2.y, q2.z, q2.w, tmin);
         if (kCount) n.box += 2;
@@ -983,7 +1007,7 @@ __device__ Hit CoopTrace(const SceneView& sv, const RayF& r, OpState* st, uint32
         if (!dL & !dR) sp0 = ComputeTask(X);
       } else {
         if (kCount) n.comp++;
-        const Hit L{st[X].lt, st[X].lb}, R{st[X].rt, st[X].rb};
+        const Hit L{sx.lt, sx.lb}, R{sx.rt, sx.rb};
         const uint32_t typeWord = AsU(q3.z), parent = AsU(q3.w);
         const uint32_t dec = Decide(table, typeWord & 0xFFu, L, R);
         const bool retL = dec == kOutRetL, retR = (dec & ~1u) == kOutRetR;
