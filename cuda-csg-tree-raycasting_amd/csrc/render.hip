@@ -580,7 +580,7 @@ __device__ Hit Trace(const SceneView& sv, const RayF& r, uint32_t rootDesc, uint
       if (!gstamps) {
         late = true;
       } else {
-        // the workgroup's copy (LDS), or every 1024 steps the launch's own
+        // the workgroup's copy (LDS), or every 256 steps the launch's own
         uint64_t startAt = lstamps[0], drainAt = lstamps[1];
         if (drainAt == 0 && (iter & 255) == 0) {
           startAt = __hip_atomic_load(gstamps, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
