@@ -148,9 +148,9 @@ size_t CoopLdsBytes(int32_t ops, uint32_t* queueCap);
 #else
 #define CSG_HOST_DEVICE
 #endif
-constexpr size_t kCoopOpStateBytes = 24;
+constexpr size_t kCoopOpStateBytes = 20;
 CSG_HOST_DEVICE inline size_t CoopWaveLds(int32_t ops, uint32_t queueCap) {
-  const size_t bytes = static_cast<size_t>(ops) * kCoopOpStateBytes + static_cast<size_t>(queueCap) * 4;
+  const size_t bytes = static_cast<size_t>(ops) * kCoopOpStateBytes + static_cast<size_t>(queueCap) * 2;
   return (bytes + 15) & ~static_cast<size_t>(15);
 }
 size_t CoopLaunchLds(const RenderParams& p);

@@ -903,14 +903,21 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(7))) 
 // happens exactly once, so results and event counts are identical.  At most
 // one task per operator is alive at a time, so task state is indexed by
 // operator.
+//
+// The count of child records still to arrive (0-2) lives in bits 11-12 of
+// the left record's word `lb`, which no record uses: the right child's
+// delivery subtracts one count, the left child's adds its record bits less
+// one count (the word's record bits are clear whenever a left delivery is
+// awaited), both as one atomic, so exactly one of two deliveries sees the
+// count reach zero.  Readers mask the bits off.
 struct OpState {
   float tmin;
-  uint32_t pending;  // child records still to arrive
   float lt;
-  uint32_t lb;
+  uint32_t lb;  // left record bits | pending count << 11
   float rt;
   uint32_t rb;
 };
+constexpr uint32_t kPendOne = 1u << 11, kPendMask = 3u << 11;
 
 // Orders the LDS accesses of the lanes of one wave (waves of a cooperative
 // workgroup run independently, so no workgroup barrier).
@@ -920,7 +927,7 @@ __device__ __forceinline__ void WaveSync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-static_assert(sizeof(OpState) == kCoopOpStateBytes, "cooperative LDS sizing assumes 24-byte operator states");
+static_assert(sizeof(OpState) == kCoopOpStateBytes, "cooperative LDS sizing assumes 20-byte operator states");
 
 __device__ __forceinline__ uint32_t EnterTask(uint32_t op) { return op << 1; }
 __device__ __forceinline__ uint32_t ComputeTask(uint32_t op) { return (op << 1) | 1u; }
@@ -936,7 +943,7 @@ __device__ __forceinline__ uint32_t LanesBelow(uint64_t mask) {
 // queues a second one in the LDS ring, from which idle lanes take work; the
 // root's result is found with a ballot.
 template <bool kFast, bool kCount>
-__device__ Hit CoopTrace(const SceneView& sv, const RayF& r, OpState* st, uint32_t* q, uint32_t qmask,
+__device__ Hit CoopTrace(const SceneView& sv, const RayF& r, OpState* st, uint16_t* q, uint32_t qmask,
                          const uint16_t* table, int lane, Counts& n) {
   uint32_t mine = lane == 0 ? EnterTask(0) : kNone;
   if (lane == 0) st[0].tmin = 0.0f;
@@ -986,10 +993,9 @@ __device__ Hit CoopTrace(const SceneView& sv, const RayF& r, OpState* st, uint32
           }
         }
         const bool dL = gL & !pL, dR = gR & !pR;
-        if (!dL) {
-          st[X].lt = L.t;
-          st[X].lb = L.bits;
-        }
+        const uint32_t pend = static_cast<uint32_t>(dL) + static_cast<uint32_t>(dR);
+        if (!dL) st[X].lt = L.t;
+        st[X].lb = (dL ? 0u : L.bits) | pend * kPendOne;
         if (!dR) {
           st[X].rt = R.t;
           st[X].rb = R.bits;
@@ -1003,11 +1009,10 @@ __device__ Hit CoopTrace(const SceneView& sv, const RayF& r, OpState* st, uint32
           if (dL) sp1 = EnterTask(DescIndex(rDesc));
           else sp0 = EnterTask(DescIndex(rDesc));
         }
-        st[X].pending = static_cast<uint32_t>(dL) + static_cast<uint32_t>(dR);
         if (!dL & !dR) sp0 = ComputeTask(X);
       } else {
         if (kCount) n.comp++;
-        const Hit L{sx.lt, sx.lb}, R{sx.rt, sx.rb};
+        const Hit L{sx.lt, sx.lb & ~kPendMask}, R{sx.rt, sx.rb};
         const uint32_t typeWord = AsU(q3.z), parent = AsU(q3.w);
         const uint32_t dec = Decide(table, typeWord & 0xFFu, L, R);
         const bool retL = dec == kOutRetL, retR = (dec & ~1u) == kOutRetR;
@@ -1029,7 +1034,8 @@ __device__ Hit CoopTrace(const SceneView& sv, const RayF& r, OpState* st, uint32
             sp0 = ComputeTask(X);
           } else {
             st[DescIndex(d)].tmin = tmin;
-            st[X].pending = 1;
+            // one record to come; a left one arrives into clear bits
+            st[X].lb = (loopL ? 0u : L.bits) | kPendOne;
             sp0 = EnterTask(DescIndex(d));
           }
         } else {
@@ -1038,14 +1044,15 @@ __device__ Hit CoopTrace(const SceneView& sv, const RayF& r, OpState* st, uint32
           if (parent == kNone) {
             fin = true;
           } else {
-            if (typeWord & 0x100u) {
+            const bool right = (typeWord & 0x100u) != 0;
+            if (right) {
               st[parent].rt = out.t;
               st[parent].rb = out.bits;
             } else {
               st[parent].lt = out.t;
-              st[parent].lb = out.bits;
             }
-            if (atomicSub(&st[parent].pending, 1u) == 1u) sp0 = ComputeTask(parent);
+            const uint32_t old = atomicAdd(&st[parent].lb, (right ? 0u : out.bits) - kPendOne);
+            if ((old & kPendMask) == kPendOne) sp0 = ComputeTask(parent);
           }
         }
       }
@@ -1060,7 +1067,7 @@ __device__ Hit CoopTrace(const SceneView& sv, const RayF& r, OpState* st, uint32
     // keep the first spawned task, queue the second
     mine = sp0;
     const uint64_t bq = __ballot(sp1 != kNone);
-    if (sp1 != kNone) q[(tail + LanesBelow(bq)) & qmask] = sp1;
+    if (sp1 != kNone) q[(tail + LanesBelow(bq)) & qmask] = static_cast<uint16_t>(sp1);
     tail += static_cast<uint32_t>(__popcll(bq));
     WaveSync();
   }
@@ -1079,7 +1086,7 @@ __global__ void __launch_bounds__(64) CoopKernel(RenderParams p) {
   const int lane = threadIdx.x;
   if (lane < kDecisionEntries) table[lane] = kDecisions.v[lane];
   OpState* st = reinterpret_cast<OpState*>(ldsBase);
-  uint32_t* q = reinterpret_cast<uint32_t*>(st + p.coopOps);
+  uint16_t* q = reinterpret_cast<uint16_t*>(st + p.coopOps);
   const uint32_t qmask = p.coopQueueCap - 1;
   __syncthreads();
   const uint32_t listed = __hip_atomic_load(p.queue + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -1200,6 +1207,7 @@ int32_t MaxSupportedDepth() { return kMaxDepth; }
 size_t CoopLaunchLds(const RenderParams& p) { return CoopWaveLds(p.coopOps, p.coopQueueCap); }
 
 size_t CoopLdsBytes(int32_t ops, uint32_t* queueCap) {
+  if (ops > 32767) return 0;  // ring entries hold op << 1 | kind in 16 bits
   uint32_t cap = 1;
   while (cap < static_cast<uint32_t>(ops > 0 ? ops : 1)) cap <<= 1;
   *queueCap = cap;
