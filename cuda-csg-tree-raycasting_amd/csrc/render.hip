@@ -599,8 +599,12 @@ __device__ Hit Trace(const SceneView& sv, const RayF& r, uint32_t rootDesc, uint
   return L;
 }
 
-__device__ __forceinline__ void Normalize3(float& x, float& y, float& z) {  // Float3Utils.cuh:33-37
-  const float inv = 1.0f / sqrtf(x * x + y * y + z * z);
+// Float3Utils.cuh:33-37.  Fast mode takes the hardware reciprocal square
+// root (about 1 ulp) in place of a correctly rounded root and division.
+template <bool kFast>
+__device__ __forceinline__ void Normalize3(float& x, float& y, float& z) {
+  const float len2 = x * x + y * y + z * z;
+  const float inv = kFast ? __builtin_amdgcn_rsqf(len2) : 1.0f / sqrtf(len2);
   x = inv * x;
   y = inv * y;
   z = inv * z;
@@ -609,6 +613,7 @@ __device__ __forceinline__ void Normalize3(float& x, float& y, float& z) {  // F
 // Ray for pixel (x, y) (RaycastingKernels.cu:11-27, Ray.cuh:12-18).  The
 // reference samples the pixel at offset 0.5; with jitter the offsets come
 // from JitterHash.
+template <bool kFast>
 __device__ __forceinline__ RayF MakeRay(const Job& jb, int x, int y, float ox, float oy) {
   const float width = static_cast<float>(jb.width), height = static_cast<float>(jb.height);
   const float u = (static_cast<float>(x) + ox) / (width - 1);
@@ -618,8 +623,9 @@ __device__ __forceinline__ RayF MakeRay(const Job& jb, int x, int y, float ox, f
   float dx = jb.right[0] * nx + jb.up[0] * ny + jb.forward[0];
   float dy = jb.right[1] * nx + jb.up[1] * ny + jb.forward[1];
   float dz = jb.right[2] * nx + jb.up[2] * ny + jb.forward[2];
-  Normalize3(dx, dy, dz);
-  const float invLen = 1.0f / sqrtf(dx * dx + dy * dy + dz * dz);
+  Normalize3<kFast>(dx, dy, dz);
+  const float len2 = dx * dx + dy * dy + dz * dz;
+  const float invLen = kFast ? __builtin_amdgcn_rsqf(len2) : 1.0f / sqrtf(len2);
   RayF r;
   r.ox = jb.pos[0];
   r.oy = jb.pos[1];
@@ -627,9 +633,9 @@ __device__ __forceinline__ RayF MakeRay(const Job& jb, int x, int y, float ox, f
   r.dx = dx * invLen;
   r.dy = dy * invLen;
   r.dz = dz * invLen;
-  r.ix = 1.0f / r.dx;
-  r.iy = 1.0f / r.dy;
-  r.iz = 1.0f / r.dz;
+  r.ix = kFast ? __builtin_amdgcn_rcpf(r.dx) : 1.0f / r.dx;
+  r.iy = kFast ? __builtin_amdgcn_rcpf(r.dy) : 1.0f / r.dy;
+  r.iz = kFast ? __builtin_amdgcn_rcpf(r.dz) : 1.0f / r.dz;
   r.nox = -r.ox * r.ix;
   r.noy = -r.oy * r.iy;
   r.noz = -r.oz * r.iz;
@@ -638,6 +644,17 @@ __device__ __forceinline__ RayF MakeRay(const Job& jb, int x, int y, float ox, f
 
 // Hit details (:183-200, :338-372, :436-457) and Phong (LightningKernel
 // :62-110), writing output entry `pix`.
+// x^n for x >= 0 (or NaN): fast mode as exp2(n log2 x) on the hardware's
+// base-2 exponential and logarithm, with powf's values at x = 0 (0 for
+// n > 0, 1 for n = 0, +inf for n < 0).
+template <bool kFast>
+__device__ __forceinline__ float PowNonNeg(float x, float n) {
+  if (!kFast) return powf(x, n);
+  const float p = __builtin_amdgcn_exp2f(n * __builtin_amdgcn_logf(x));
+  return x > 0.0f ? p : (x == 0.0f ? (n > 0.0f ? 0.0f : (n == 0.0f ? 1.0f : __builtin_inff())) : p);
+}
+
+template <bool kFast>
 __device__ __forceinline__ void ShadeAndStore(const Job& jb, const SceneView& sv, const RayF& r, Hit h, size_t pix) {
   const uint32_t flags = Flags(h.bits);
   float4 colour = make_float4(jb.bg[0], jb.bg[1], jb.bg[2], jb.bg[3]);
@@ -654,7 +671,7 @@ __device__ __forceinline__ void ShadeAndStore(const Job& jb, const SceneView& sv
       Nx = px - a.x;
       Ny = py - a.y;
       Nz = pz - a.z;
-      Normalize3(Nx, Ny, Nz);
+      Normalize3<kFast>(Nx, Ny, Nz);
     } else if (type == kCylinderT) {
       const float4 b = sv.primB[prim];
       if (flags & kFlag1) {
@@ -669,7 +686,7 @@ __device__ __forceinline__ void ShadeAndStore(const Job& jb, const SceneView& sv
         Nx = (px - Cx) - m * b.x;
         Ny = (py - Cy) - m * b.y;
         Nz = (pz - Cz) - m * b.z;
-        Normalize3(Nx, Ny, Nz);
+        Normalize3<kFast>(Nx, Ny, Nz);
       }
     } else {
       const float half = a.w / 2;
@@ -677,7 +694,7 @@ __device__ __forceinline__ void ShadeAndStore(const Job& jb, const SceneView& sv
       Nx = TruncInt((px - a.x) / half * bias);
       Ny = TruncInt((py - a.y) / half * bias);
       Nz = TruncInt((pz - a.z) / half * bias);
-      Normalize3(Nx, Ny, Nz);
+      Normalize3<kFast>(Nx, Ny, Nz);
     }
     if (flags & kFlip) { Nx = -Nx; Ny = -Ny; Nz = -Nz; }
     if (flags & kExit) { Nx = -Nx; Ny = -Ny; Nz = -Nz; }
@@ -685,15 +702,15 @@ __device__ __forceinline__ void ShadeAndStore(const Job& jb, const SceneView& sv
     const float4 col = jb.sc.primC[prim];
     const float Lx = jb.light[0], Ly = jb.light[1], Lz = jb.light[2];
     float Vx = jb.pos[0] - px, Vy = jb.pos[1] - py, Vz = jb.pos[2] - pz;
-    Normalize3(Vx, Vy, Vz);
+    Normalize3<kFast>(Vx, Vy, Vz);
     // reflect(-L, N) (Float3Utils.cuh:39-44): a - 2(a.n)n with n = normalize(N)
     float nnx = Nx, nny = Ny, nnz = Nz;
-    Normalize3(nnx, nny, nnz);
+    Normalize3<kFast>(nnx, nny, nnz);
     const float ax = -Lx, ay = -Ly, az = -Lz;
     const float s2 = 2.0f * (ax * nnx + ay * nny + az * nnz);
     const float Rx = ax - s2 * nnx, Ry = ay - s2 * nny, Rz = az - s2 * nnz;
     const float diff = fmaxf(Nx * Lx + Ny * Ly + Nz * Lz, 0.0f);
-    const float spec = powf(fmaxf(Vx * Rx + Vy * Ry + Vz * Rz, 0.0f), jb.shininess);
+    const float spec = PowNonNeg<kFast>(fmaxf(Vx * Rx + Vy * Ry + Vz * Rz, 0.0f), jb.shininess);
     const float kdiff = jb.kd * diff * 1.0f;
     const float kspec = jb.ks * spec * 1.0f;
     const float ka = jb.ka * 1.0f;
@@ -749,7 +766,7 @@ __device__ __forceinline__ void RenderTile(const Job& jb, const SceneView& sv, u
     ox = JitterUnit(JitterHash(jb.seed, pixel, sample, 0));
     oy = JitterUnit(JitterHash(jb.seed, pixel, sample, 1));
   }
-  const RayF r = MakeRay(jb, x, y, ox, oy);
+  const RayF r = MakeRay<kFast>(jb, x, y, ox, oy);
   Counts n{0, 0, 0, 0, 0, 0};
   const size_t out = static_cast<size_t>(pixel) * static_cast<size_t>(jb.spp) + sample;
   bool exceeded = false;
@@ -768,7 +785,7 @@ __device__ __forceinline__ void RenderTile(const Job& jb, const SceneView& sv, u
     h = Trace<kFast, kCount>(sv, r, jb.sc.rootDesc, slab, stride, cap, table, n, 0x7FFFFFFF, lstamps, nullptr, 0,
                              exceeded);
   }
-  ShadeAndStore(jb, sv, r, h, out);
+  ShadeAndStore<kFast>(jb, sv, r, h, out);
   if (kCount && jb.counters) {
     int32_t* c = jb.counters + 6 * out;
     c[0] = n.go; c[1] = n.comp; c[2] = n.box; c[3] = n.sph; c[4] = n.cyl; c[5] = n.cub;
@@ -1120,7 +1137,7 @@ __global__ void __launch_bounds__(64) CoopKernel(RenderParams p) {
       ox = JitterUnit(JitterHash(jb.seed, pixel, sample, 0));
       oy = JitterUnit(JitterHash(jb.seed, pixel, sample, 1));
     }
-    const RayF r = MakeRay(jb, x, y, ox, oy);
+    const RayF r = MakeRay<kFast>(jb, x, y, ox, oy);
     Counts n{0, 0, 0, 0, 0, 0};
     const Hit h = CoopTrace<kFast, kCount>(sv, r, st, q, qmask, table, lane, n);
     if (kCount && jb.counters) {
@@ -1131,7 +1148,7 @@ __global__ void __launch_bounds__(64) CoopKernel(RenderParams p) {
       int32_t* c = jb.counters + 6 * static_cast<size_t>(out);
       for (int i = 0; i < 6; i++) c[i] = v[i];
     }
-    ShadeAndStore(jb, sv, r, h, out);
+    ShadeAndStore<kFast>(jb, sv, r, h, out);
   }
   // The last wave out adds the launch's heavy rays to the stream's running
   // total (words 6-7) and returns the heavy-ray counters to zero.

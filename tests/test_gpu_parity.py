@@ -237,3 +237,32 @@ def test_random_tree_exact_matches_fp32_oracle(gpu, seed):
     assert np.array_equal(g["flags"], o["flags"])
     assert np.array_equal(g["depth"].view(np.uint32), o["depth"].view(np.uint32))
     assert np.array_equal(g["counters"], o["counters"])
+
+
+# fast vs exact RGBA on the same hit: half an 8-bit display step.  The difference is set by fast
+# mode's hit-point rounding (~1e-6 relative), which moves a small sphere's normal by about
+# depth error x t / r (1e-4 for r = 0.05 at t = 5) and is amplified by the specular power; the
+# hardware rsq/rcp/exp2/log2 of fast shading add ~1e-6.  A wrong power at x = 0 or a missing
+# normalisation shows as ~0.1.
+FAST_RGBA_TOL = 2e-3
+
+
+@pytest.mark.parametrize("setup", [0, 1])
+@pytest.mark.parametrize("name", scene_names())
+def test_fast_shading_close_to_exact(gpu, name, setup):
+    """Fast mode normalises with the hardware reciprocal square root and
+    takes the specular power as exp2(n log2 x); on every ray where fast and
+    exact mode hit the same primitive with the same flags, RGBA agrees within
+    FAST_RGBA_TOL, and misses carry the background exactly."""
+    _label(name, f"{setup} fast vs exact rgba")
+    sname, cam, light, phong, cam6, l3, ph8 = setups()[setup]
+    text = scene_text(name)
+    f = gpu_render(text, cam, light, phong, 160, 90, C.MODE_FAST)
+    e = gpu_render(text, cam, light, phong, 160, 90, C.MODE_EXACT)
+    same = (f["prim_id"] == e["prim_id"]) & (f["flags"] == e["flags"])
+    assert same.mean() > 0.99
+    d = np.abs(f["rgba"] - e["rgba"])[same]
+    print(f"max |rgba fast - exact| over {int(same.sum())} rays: {d.max():.2e}")
+    assert d.max() <= FAST_RGBA_TOL
+    miss = same & (f["prim_id"] < 0)
+    assert np.array_equal(f["rgba"][miss], e["rgba"][miss])
