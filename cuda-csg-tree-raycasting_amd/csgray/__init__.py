@@ -68,7 +68,7 @@ class LaunchInfo(ctypes.Structure):
     _fields_ = [("block", ctypes.c_int32), ("grid", ctypes.c_int32), ("lds_bytes", ctypes.c_int64),
                 ("scene_in_lds", ctypes.c_int32), ("compute_units", ctypes.c_int32), ("lds_limit", ctypes.c_int64),
                 ("coop_budget", ctypes.c_int32), ("coop_grid", ctypes.c_int32), ("coop_block", ctypes.c_int32),
-                ("reserved0", ctypes.c_int32), ("coop_lds_bytes", ctypes.c_int64)]
+                ("coop_grace", ctypes.c_int32), ("coop_lds_bytes", ctypes.c_int64)]
 
 
 class BatchTuning(ctypes.Structure):
@@ -282,9 +282,9 @@ class Scene:
         _check(lib().csg_scene_export(self.handle, links.ctypes.data, boxes.ctypes.data, prims.ctypes.data))
         return links, boxes, prims
 
-    def plan(self, mode=MODE_FAST, block=0, placement=0, coop=0) -> LaunchInfo:
+    def plan(self, mode=MODE_FAST, block=0, placement=0, coop=0, coop_grace=0) -> LaunchInfo:
         info = LaunchInfo()
-        opts = RenderOpts(mode, block, placement, 0, None, None, 0, 0, 0, coop)
+        opts = RenderOpts(mode, block, placement, 0, None, None, 0, 0, 0, coop, 0, coop_grace)
         _check(lib().csg_render_plan(self.handle, ctypes.byref(opts), ctypes.byref(info)))
         return info
 

@@ -118,7 +118,7 @@ csg_status Stage(csg_batch* b) {
     if (e != hipSuccess) return FailHip(e, "choosing the launch shape");
     csgray::FitGridToTiles(&ch.shape, base);
     p.totalWaves = static_cast<uint32_t>(ch.shape.grid) * static_cast<uint32_t>(ch.shape.block / 64);
-    PrepareCoop(b->cfg, maxOps, ch.heavy, &p);
+    PrepareCoop(b->cfg, maxOps, ch.heavy, ch.shape, &p);
   }
   return CSG_OK;
 }

@@ -144,7 +144,16 @@ CoopPlan PlanCoop(const LaunchConfig& cfg, int32_t maxOps) {
   return c;
 }
 
-void PrepareCoop(const LaunchConfig& cfg, int32_t maxOps, uint2* heavy, RenderParams* p) {
+// Default tail grace for a launch shape: the time a tail takes grows with
+// the tiles in flight when the queue runs dry, i.e. with the waves per CU.
+// A single-scene launch with the scene in LDS runs one workgroup per CU:
+// 50 % at 512 lanes (8 waves per CU), 100 % at 1024 (measured best at each
+// on the cheese views and random trees).  Other launches keep 50 %.
+int32_t DefaultCoopGrace(const LaunchShape& shape) {
+  return shape.sceneInLds ? kDefaultCoopGrace * std::max(1, shape.block / 512) : kDefaultCoopGrace;
+}
+
+void PrepareCoop(const LaunchConfig& cfg, int32_t maxOps, uint2* heavy, const LaunchShape& shape, RenderParams* p) {
   p->budget = 0;
   if (!heavy) return;
   const CoopPlan c = PlanCoop(cfg, maxOps);
@@ -156,7 +165,7 @@ void PrepareCoop(const LaunchConfig& cfg, int32_t maxOps, uint2* heavy, RenderPa
   p->coopQueueCap = c.queueCap;
   p->coopWaves = c.waves;
   p->coopEager = cfg.coopEager ? 1 : 0;
-  p->coopGrace = cfg.coopGrace > 0 ? cfg.coopGrace : (cfg.coopGrace < 0 ? 0 : kDefaultCoopGrace);
+  p->coopGrace = cfg.coopGrace > 0 ? cfg.coopGrace : (cfg.coopGrace < 0 ? 0 : DefaultCoopGrace(shape));
 }
 
 }  // namespace rt
@@ -506,6 +515,8 @@ csg_status csg_render_plan(const csg_scene* s, const csg_render_opts* opts, csg_
   info->coop_budget = c.budget;
   info->coop_grid = static_cast<int32_t>(c.waves);
   info->coop_block = 64;
+  info->coop_grace = c.budget > 0 ? (cfg.coopGrace > 0 ? cfg.coopGrace : (cfg.coopGrace < 0 ? 0 : DefaultCoopGrace(shape)))
+                                  : 0;
   info->coop_lds_bytes = static_cast<int64_t>(c.ldsBytes);
   return CSG_OK;
 }
@@ -564,7 +575,7 @@ csg_status csg_render_ex(const csg_scene* s, const csg_camera* cam, const csg_li
     const csg_status hsd = HeavyFor(dev, hs, &heavy);
     if (hsd != CSG_OK) return hsd;
   }
-  PrepareCoop(cfg, s->dev.nOps, heavy, &p);
+  PrepareCoop(cfg, s->dev.nOps, heavy, shape, &p);
   e = csgray::LaunchRender(p, cfg, shape, hs);
   if (e != hipSuccess) return FailHip(e, "render launch");
   return CSG_OK;
@@ -635,7 +646,7 @@ csg_status csg_render_batch(const csg_job* jobs, int32_t njobs, const csg_render
     }
     int32_t maxOps = 1;
     for (int32_t i = 0; i < n; i++) maxOps = std::max(maxOps, jobs[start + i].scene->dev.nOps);
-    PrepareCoop(cfg, maxOps, heavy, &p);
+    PrepareCoop(cfg, maxOps, heavy, shape, &p);
     e = csgray::LaunchRender(p, cfg, shape, hs);
     if (e != hipSuccess) return FailHip(e, "batched render launch");
   }

@@ -179,7 +179,7 @@ void Dispatch(csg_server* s, Slot& sl) {
     p.totalWaves = static_cast<uint32_t>(shape.grid) * static_cast<uint32_t>(shape.block / 64);
     int32_t maxOps = 1;
     for (int64_t tk : take) maxOps = std::max(maxOps, s->scenes[s->reqs[tk].req.scene_id]->dev.nOps);
-    PrepareCoop(s->launch, maxOps, heavy, &p);
+    PrepareCoop(s->launch, maxOps, heavy, shape, &p);
     if (csgray::LaunchRender(p, s->launch, shape, sl.stream) != hipSuccess) st = CSG_E_HIP;
   }
   if (st == CSG_OK && hipEventRecord(sl.done, sl.stream) != hipSuccess) st = CSG_E_HIP;

@@ -158,6 +158,10 @@ def test_launch_plan(gpu):
     assert g.scene_in_lds == 0
     lds = s.plan(placement=C.PLACE_LDS)
     assert lds.scene_in_lds == 1 and lds.lds_bytes <= lds.lds_limit
+    # the largest workgroup whose stacks and scene copy fit, and a tail grace scaled to its waves per CU
+    assert auto.scene_in_lds == 1 and auto.block == 1024 and auto.coop_grace == 100
+    assert s.plan(block=512).coop_grace == 50 and g.coop_grace == 50
+    assert s.plan(coop_grace=30).coop_grace == 30 and s.plan(coop=-1).coop_grace == 0
 
 
 def test_queue_reuse_across_sizes_scenes_and_streams(gpu):
