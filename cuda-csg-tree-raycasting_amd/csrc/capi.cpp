@@ -563,7 +563,7 @@ csg_status csg_render_ex(const csg_scene* s, const csg_camera* cam, const csg_li
   p.tileOrder = opts ? opts->tile_order : nullptr;
   p.morton = cfg.morton ? 1 : 0;
   csgray::LaunchShape shape;
-  e = csgray::ChooseShape(s->dev.opDepth, SceneBytes(s), cfg, &shape);
+  e = csgray::ChooseShape(s->dev.opDepth, SceneBytes(s), cfg, &shape, 0, p.numTiles);
   if (e != hipSuccess) return FailHip(e, "choosing the launch shape");
   csgray::FitGridToTiles(&shape, p.numTiles);
   p.totalWaves = static_cast<uint32_t>(shape.grid) * static_cast<uint32_t>(shape.block / 64);

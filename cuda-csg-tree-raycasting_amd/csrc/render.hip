@@ -1243,7 +1243,7 @@ hipError_t DeviceShapeLimits(int* computeUnits, size_t* ldsPerBlock) {
 }
 
 hipError_t ChooseShape(int32_t depthIn, size_t sceneBytes, const LaunchConfig& cfg, LaunchShape* shape,
-                       int32_t batchJobs) {
+                       int32_t batchJobs, int64_t tiles) {
   const DeviceLimits* lp = CurrentLimits();
   if (!lp) return hipErrorNoDevice;
   const DeviceLimits& lim = *lp;
@@ -1257,11 +1257,16 @@ hipError_t ChooseShape(int32_t depthIn, size_t sceneBytes, const LaunchConfig& c
   bool lds = false;
   int block = cfg.block > 0 ? cfg.block : 0;
   if (cfg.sceneInLds != 0 && sceneBytes > 0) {
-    // the largest workgroup that fits: most waves sharing one scene copy
-    // (1024 lanes, 4 waves per SIMD, measured 4-25 % faster than 512 on the
-    // cheese views and random trees, with the tail grace scaled to match)
+    // The largest workgroup that fits: most waves sharing one scene copy.
+    // 1024 lanes (4 waves per SIMD) hold twice the tiles in flight and take
+    // each about twice as long, so they pay only on frames of at least 5
+    // tiles per wave of that grid: there they measured 4-25 % faster than
+    // 512 on heavy scenes (cheese views, random trees at 1920x1080); below
+    // it up to 25 % slower, and on light scenes a little slower at any size.
+    const bool wide = tiles <= 0 || tiles >= int64_t{5} * cus * 16;
     for (int b : {1024, 512, 256}) {
       if (block && b != block) continue;
+      if (!block && b == 1024 && !wide) continue;
       if (depth * b * 8 + sceneBytes + kStaticLds <= ldsCap) {
         lds = true;
         block = b;

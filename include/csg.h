@@ -241,7 +241,10 @@ csg_status csg_batch_autotune(csg_batch* batch, int32_t reps, void* stream, csg_
 csg_status csg_batch_free(csg_batch* batch);
 
 /* Launch geometry csg_render_ex would use for this scene and options on the
- * current device (no reference counterpart; for tests and tuning). */
+ * current device, for a large frame (no reference counterpart; for tests and
+ * tuning).  A single-scene launch with the scene in LDS takes 1024-lane
+ * workgroups when they fit and the frame has at least 5 tiles of 8x8 per wave
+ * of that grid (1920x1080 on 256 CUs); smaller frames take 512 lanes. */
 typedef struct csg_launch_info {
   int32_t block;          /* threads per workgroup */
   int32_t grid;           /* workgroups (persistent: waves pull 8x8 tiles from a queue) */
