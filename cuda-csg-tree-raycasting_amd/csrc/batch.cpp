@@ -53,7 +53,7 @@ csg_status AllocChunk(Chunk& c, int32_t cap) {
   if (e == hipSuccess) e = hipHostMalloc(&c.hostJobs, sizeof(csgray::Job) * cap, hipHostMallocDefault);
   if (e == hipSuccess) e = hipMalloc(&c.queue, 256);
   if (e == hipSuccess) e = hipMemset(c.queue, 0, 256);
-  if (e == hipSuccess) e = hipMalloc(&c.heavy, sizeof(uint2) * csgray::kHeavyCap);
+  if (e == hipSuccess) e = hipMalloc(&c.heavy, csgray::kHeavyBytes);
   if (e != hipSuccess) return e == hipErrorOutOfMemory ? Fail(CSG_E_OOM, "batch allocation failed") : FailHip(e, "batch allocation");
   c.cap = cap;
   return CSG_OK;

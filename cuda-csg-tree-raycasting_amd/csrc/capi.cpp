@@ -110,7 +110,7 @@ csg_status HeavyFor(int dev, hipStream_t stream, uint2** out) {
     return CSG_OK;
   }
   void* h = nullptr;
-  const hipError_t e = hipMalloc(&h, sizeof(uint2) * csgray::kHeavyCap);
+  const hipError_t e = hipMalloc(&h, csgray::kHeavyBytes);
   if (e != hipSuccess) return e == hipErrorOutOfMemory ? Fail(CSG_E_OOM, "heavy-ray list allocation failed")
                                                        : FailHip(e, "hipMalloc");
   g_heavy[{dev, stream}] = static_cast<uint2*>(h);
@@ -161,6 +161,7 @@ void PrepareCoop(const LaunchConfig& cfg, int32_t maxOps, uint2* heavy, const La
   p->budget = c.budget;
   p->heavy = heavy;
   p->heavyCap = kHeavyCap;
+  p->resume = reinterpret_cast<uint32_t*>(heavy + kHeavyCap);
   p->coopOps = maxOps;
   p->coopQueueCap = c.queueCap;
   p->coopWaves = c.waves;

@@ -68,6 +68,10 @@ struct RenderParams {
   int32_t coopOps;            // operator-state slots per wave (max operators of the launch)
   uint32_t coopQueueCap;      // task ring entries (power of two >= coopOps)
   uint32_t coopWaves;         // waves in the cooperative grid
+  // Resume area (kResumeBytes, right after the heavy list): for the first
+  // ResumeCap() handed-over rays, the state machine's state at hand-over, so
+  // the cooperative kernel continues the ray instead of restarting it.
+  uint32_t* resume;
 };
 // queue words: [0] next tile, [1] tile waves done, [2] heavy rays, [3] next heavy ray, [4] coop waves done,
 // [6..7] heavy rays handed over on this stream so far (64-bit); on their own cache line, [32..33] time
@@ -138,6 +142,16 @@ constexpr int32_t kMaxBatchJobs = 1024;
 constexpr int32_t kDefaultBudget = 384;
 constexpr int32_t kDefaultCoopGrace = 50;
 constexpr uint32_t kHeavyCap = 1u << 22;
+// Resume records of handed-over rays: 20 words each (current operator,
+// action, stack slots, tmin, the L/R/held records, the event counts so far),
+// then per ray and stack slot the slot's entry (8 B) and the operator it
+// belongs to (4 B).  kResumeFlag in a heavy-list entry's job word marks a
+// ray with a record.
+constexpr size_t kResumeBytes = size_t{64} << 20;
+constexpr uint32_t kResumeWords = 20;
+constexpr uint32_t kResumeFlag = 1u << 31;
+// Bytes of one heavy list together with its resume area.
+constexpr size_t kHeavyBytes = sizeof(uint32_t) * 2 * kHeavyCap + kResumeBytes;
 
 // LDS bytes the cooperative kernel needs for scenes of at most `ops`
 // operators (0 when it cannot run: the caller then disables deferral).

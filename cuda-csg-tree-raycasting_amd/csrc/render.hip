@@ -428,10 +428,20 @@ __device__ __forceinline__ Hit Intersect(const SceneView& sv, const RayF& r, uin
 // frame's tail.  Lanes of a wave enter together and step together, so
 // `iter` is the same in every active lane and the check (at `budget` steps,
 // then every 128; the global stamps every 256) is wave-uniform.
+// The state machine's state when a ray is handed over: the operator it
+// stands on, the next action, tmin, the records and the stack depth (the
+// stack itself stays in the caller's slab).
+struct TraceState {
+  uint32_t cur, action;
+  int slots;
+  float tmin;
+  Hit L, R, held;
+};
+
 template <bool kFast, bool kCount>
 __device__ Hit Trace(const SceneView& sv, const RayF& r, uint32_t rootDesc, uint2* stk, int stride, int cap,
                      const uint16_t* table, Counts& n, int budget, const uint64_t* lstamps,
-                     const uint64_t* gstamps, int grace, bool& exceeded) {
+                     const uint64_t* gstamps, int grace, bool& exceeded, TraceState* ts = nullptr) {
   exceeded = false;
   if (DescType(rootDesc) >= kSphereT) {
     // A primitive root: GoTo intersects it with tmin 0 and the state machine
@@ -447,7 +457,7 @@ __device__ Hit Trace(const SceneView& sv, const RayF& r, uint32_t rootDesc, uint
   float tmin = 0.0f;
   // The operator the machine stands on.  Initially the reference's virtual
   // parent of the root, whose left child is the root.
-  uint32_t opType = 0, opParent = kNone, lDesc = rootDesc, rDesc = 0;
+  uint32_t opType = 0, opParent = kNone, lDesc = rootDesc, rDesc = 0, cur = kNone;
   // At the top of the loop action is GotoLft, GotoRgh, Compute, LoadLft or
   // LoadRgh.
   int action = kGotoLft;
@@ -477,6 +487,7 @@ __device__ Hit Trace(const SceneView& sv, const RayF& r, uint32_t rootDesc, uint
         action = (e.y & kSlotMask) == kSlotLoadLft ? kLoadLft : kLoadRgh;
       } else {
         const uint32_t k = DescIndex(d);
+        cur = k;
         const float4 q0 = sv.ops[4 * k + 0];
         const float4 q1 = sv.ops[4 * k + 1];
         const float4 q2 = sv.ops[4 * k + 2];
@@ -558,6 +569,7 @@ __device__ Hit Trace(const SceneView& sv, const RayF& r, uint32_t rootDesc, uint
         sp = top;
         held = Hit{__uint_as_float(e.x), e.y & ~kSlotMask};
         const uint32_t code = e.y & kSlotMask;
+        cur = opParent;
         lDesc = AsU(pq.x);
         rDesc = AsU(pq.y);
         opType = AsU(pq.z) & 0xFFu;
@@ -595,6 +607,7 @@ __device__ Hit Trace(const SceneView& sv, const RayF& r, uint32_t rootDesc, uint
     exceeded = !done & !aborted & late;
     done = done | aborted | exceeded | (iter >= kIterCap);
   }
+  if (ts && exceeded) *ts = TraceState{cur, static_cast<uint32_t>(action), sp / stride, tmin, L, R, held};
   if (aborted || (!exceeded && iter >= kIterCap)) L = Hit{-1.0f, kMissBit | kAborted | 0xFFFF0000u};
   return L;
 }
@@ -740,6 +753,20 @@ __device__ __forceinline__ uint32_t Compact1By1(uint32_t v) {
   return v;
 }
 
+// Resume area of a launch whose stacks hold `cap` slots: ResumeCap records
+// of kResumeWords words, then ResumeCap x cap stack entries, then as many
+// owners (the operator each stack slot belongs to).
+__device__ __forceinline__ uint32_t ResumeCap(const RenderParams& p, int cap) {
+  const size_t n = kResumeBytes / (kResumeWords * 4 + 12 * static_cast<size_t>(cap));
+  return n < p.heavyCap ? static_cast<uint32_t>(n) : p.heavyCap;
+}
+__device__ __forceinline__ uint2* ResumeEntries(const RenderParams& p, uint32_t rcap) {
+  return reinterpret_cast<uint2*>(p.resume + static_cast<size_t>(rcap) * kResumeWords);
+}
+__device__ __forceinline__ uint32_t* ResumeOwners(const RenderParams& p, uint32_t rcap, int cap) {
+  return reinterpret_cast<uint32_t*>(ResumeEntries(p, rcap) + static_cast<size_t>(rcap) * cap);
+}
+
 // Traces and shades the 64 rays of one tile (one per lane).
 template <bool kFast, bool kCount>
 __device__ __forceinline__ void RenderTile(const Job& jb, const SceneView& sv, uint32_t local, bool morton,
@@ -772,13 +799,45 @@ __device__ __forceinline__ void RenderTile(const Job& jb, const SceneView& sv, u
   bool exceeded = false;
   const int budget = p.budget > 0 ? p.budget : 0x7FFFFFFF;
   const uint64_t* gstamps = p.coopEager ? nullptr : reinterpret_cast<const uint64_t*>(p.queue + 32);
+  TraceState ts;
   Hit h = Trace<kFast, kCount>(sv, r, jb.sc.rootDesc, slab, stride, cap, table, n, budget, lstamps, gstamps,
-                               p.coopGrace, exceeded);
+                               p.coopGrace, exceeded, &ts);
   if (exceeded) {
-    // hand the ray to the cooperative kernel; if its list is full, finish here
+    // Hand the ray to the cooperative kernel; if its list is full, finish
+    // here.  The first ResumeCap rays leave their state, so the cooperative
+    // kernel continues them where this kernel stopped (the others restart).
     const uint32_t slot = atomicAdd(p.queue + 2, 1u);
     if (slot < p.heavyCap) {
-      p.heavy[slot] = make_uint2(jobIndex, static_cast<uint32_t>(out));
+      const uint32_t rcap = ResumeCap(p, cap);
+      const bool keep = slot < rcap && ts.slots + 2 <= 64 && ts.cur != kNone;
+      p.heavy[slot] = make_uint2(jobIndex | (keep ? kResumeFlag : 0u), static_cast<uint32_t>(out));
+      if (keep) {
+        uint32_t* rec = p.resume + static_cast<size_t>(slot) * kResumeWords;
+        rec[0] = ts.cur;
+        rec[1] = ts.action;
+        rec[2] = static_cast<uint32_t>(ts.slots);
+        rec[3] = AsU(ts.tmin);
+        rec[4] = AsU(ts.L.t);
+        rec[5] = ts.L.bits;
+        rec[6] = AsU(ts.R.t);
+        rec[7] = ts.R.bits;
+        rec[8] = AsU(ts.held.t);
+        rec[9] = ts.held.bits;
+        if (kCount) {
+          rec[10] = n.go; rec[11] = n.comp; rec[12] = n.box; rec[13] = n.sph; rec[14] = n.cyl; rec[15] = n.cub;
+        }
+        uint2* ent = ResumeEntries(p, rcap) + static_cast<size_t>(slot) * cap;
+        uint32_t* own = ResumeOwners(p, rcap, cap) + static_cast<size_t>(slot) * cap;
+        // One slot per operator on the path from the root: the top one is
+        // the current operator's when it is about to enter a child, else
+        // its parent's; each lower one belongs to the next ancestor.
+        uint32_t a = ts.action <= static_cast<uint32_t>(kGotoRgh) ? ts.cur : AsU(sv.ops[4 * ts.cur + 3].w);
+        for (int i = ts.slots - 1; i >= 0; i--) {
+          ent[i] = slab[i * stride];
+          own[i] = a;
+          if (i > 0) a = a == kNone ? kNone : AsU(sv.ops[4 * a + 3].w);
+        }
+      }
       return;
     }
     n = Counts{0, 0, 0, 0, 0, 0};
@@ -959,11 +1018,97 @@ __device__ __forceinline__ uint32_t LanesBelow(uint64_t mask) {
 // loop).  A lane keeps the first task it spawns for the next iteration and
 // queues a second one in the LDS ring, from which idle lanes take work; the
 // root's result is found with a ballot.
+//
+// With `rec` (a resume record, its stack entries `ent` and their operators
+// `own`) the ray continues from the state machine's state at hand-over: the
+// machine's stack holds one slot per operator on the path from the root,
+// each parking that operator's other child (SaveLft: the right child, an
+// operator, still to enter with the parked tmin; LoadLft: the left record
+// known, the right one to come; LoadRgh: the right record known, the left
+// one to come), and the machine's next step at the current operator becomes
+// one task.  The remaining box tests, intersections and decisions are the
+// ones the sequential machine would have made, so results and counts are
+// unchanged.
 template <bool kFast, bool kCount>
 __device__ Hit CoopTrace(const SceneView& sv, const RayF& r, OpState* st, uint16_t* q, uint32_t qmask,
-                         const uint16_t* table, int lane, Counts& n) {
-  uint32_t mine = lane == 0 ? EnterTask(0) : kNone;
-  if (lane == 0) st[0].tmin = 0.0f;
+                         const uint16_t* table, int lane, Counts& n, const uint32_t* rec = nullptr,
+                         const uint2* ent = nullptr, const uint32_t* own = nullptr) {
+  uint32_t mine = kNone;
+  if (!rec) {
+    mine = lane == 0 ? EnterTask(0) : kNone;
+    if (lane == 0) st[0].tmin = 0.0f;
+  } else {
+    const uint32_t cur = rec[0], action = rec[1];
+    const int slots = static_cast<int>(rec[2]);
+    const float tmin = __uint_as_float(rec[3]);
+    const Hit L{__uint_as_float(rec[4]), rec[5]}, R{__uint_as_float(rec[6]), rec[7]};
+    const Hit held{__uint_as_float(rec[8]), rec[9]};
+    if (kCount && lane == 0) {
+      n.go += static_cast<int32_t>(rec[10]);
+      n.comp += static_cast<int32_t>(rec[11]);
+      n.box += static_cast<int32_t>(rec[12]);
+      n.sph += static_cast<int32_t>(rec[13]);
+      n.cyl += static_cast<int32_t>(rec[14]);
+      n.cub += static_cast<int32_t>(rec[15]);
+    }
+    if (lane < slots) {
+      const uint32_t a = own[lane];
+      const uint2 e = ent[lane];
+      const uint32_t code = e.y & kSlotMask;
+      const float et = __uint_as_float(e.x);
+      const uint32_t eb = e.y & ~kSlotMask;
+      if (a != kNone) {
+        if (code == kSlotSaveLft) {
+          const uint32_t rc = DescIndex(AsU(sv.ops[4 * a + 3].y));
+          st[rc].tmin = et;
+          st[a].lb = 2 * kPendOne;
+          mine = EnterTask(rc);
+        } else if (code == kSlotLoadLft) {
+          st[a].lt = et;
+          st[a].lb = eb | kPendOne;
+        } else {
+          st[a].rt = et;
+          st[a].rb = eb;
+          st[a].lb = kPendOne;
+        }
+      }
+    }
+    WaveSync();
+    if (lane == slots) {
+      if (action <= static_cast<uint32_t>(kGotoRgh)) {
+        const bool left = action == static_cast<uint32_t>(kGotoLft);
+        const float4 q3 = sv.ops[4 * cur + 3];
+        const uint32_t d = left ? AsU(q3.x) : AsU(q3.y);
+        if (DescType(d) >= kSphereT) {
+          // re-entering a primitive child (a loop): intersect it beyond
+          // tmin and deliver its record to the current operator
+          if (kCount) n.go++;
+          const Hit h = Intersect<kFast, kCount>(sv, r, d, tmin, left ? L : R, n);
+          if (left) {
+            st[cur].lt = h.t;
+          } else {
+            st[cur].rt = h.t;
+            st[cur].rb = h.bits;
+          }
+          const uint32_t old = atomicAdd(&st[cur].lb, (left ? h.bits : 0u) - kPendOne);
+          if ((old & kPendMask) == kPendOne) mine = ComputeTask(cur);
+        } else {
+          st[DescIndex(d)].tmin = tmin;
+          mine = EnterTask(DescIndex(d));
+        }
+      } else {
+        // Compute at the current operator; LoadLft / LoadRgh first take the
+        // parked record
+        const Hit lr = action == static_cast<uint32_t>(kLoadLft) ? held : L;
+        const Hit rr = action == static_cast<uint32_t>(kLoadRgh) ? held : R;
+        st[cur].lt = lr.t;
+        st[cur].lb = lr.bits;
+        st[cur].rt = rr.t;
+        st[cur].rb = rr.bits;
+        mine = ComputeTask(cur);
+      }
+    }
+  }
   uint32_t head = 0, tail = 0;
   Hit res{-1.0f, kMissBit | kAborted | 0xFFFF0000u};
   WaveSync();
@@ -1125,7 +1270,8 @@ __global__ void __launch_bounds__(64) CoopKernel(RenderParams p) {
     k = __builtin_amdgcn_readfirstlane(k);
     if (k >= count) break;
     const uint2 e = p.heavy[k];
-    const Job& jb = kBatch ? p.jobs[__builtin_amdgcn_readfirstlane(e.x)] : p.job;
+    const uint32_t jobWord = __builtin_amdgcn_readfirstlane(e.x);
+    const Job& jb = kBatch ? p.jobs[jobWord & ~kResumeFlag] : p.job;
     const SceneView sv{jb.sc.ops, jb.sc.primA, jb.sc.primB};
     const uint32_t out = __builtin_amdgcn_readfirstlane(e.y);
     const uint32_t spp = static_cast<uint32_t>(jb.spp);
@@ -1139,7 +1285,18 @@ __global__ void __launch_bounds__(64) CoopKernel(RenderParams p) {
     }
     const RayF r = MakeRay<kFast>(jb, x, y, ox, oy);
     Counts n{0, 0, 0, 0, 0, 0};
-    const Hit h = CoopTrace<kFast, kCount>(sv, r, st, q, qmask, table, lane, n);
+    const uint32_t* rec = nullptr;
+    const uint2* ent = nullptr;
+    const uint32_t* own = nullptr;
+    if (jobWord & kResumeFlag) {
+      // the per-lane kernel's state for this ray (its slot k < ResumeCap)
+      const int cap = p.stackSlots > 0 ? p.stackSlots : 1;
+      const uint32_t rcap = ResumeCap(p, cap);
+      rec = p.resume + static_cast<size_t>(k) * kResumeWords;
+      ent = ResumeEntries(p, rcap) + static_cast<size_t>(k) * cap;
+      own = ResumeOwners(p, rcap, cap) + static_cast<size_t>(k) * cap;
+    }
+    const Hit h = CoopTrace<kFast, kCount>(sv, r, st, q, qmask, table, lane, n, rec, ent, own);
     if (kCount && jb.counters) {
       int32_t v[6] = {n.go, n.comp, n.box, n.sph, n.cyl, n.cub};
       for (int c = 0; c < 6; c++) {
