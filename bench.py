@@ -364,7 +364,8 @@ def main():
                    "rays_per_step": rays, "p50_ms": round(p50, 4), "p99_ms": round(p99, 4),
                    "wall_s": round(wall, 3), "block": plan.block, "grid": plan.grid,
                    "scene_in_lds": bool(plan.scene_in_lds), "lds_bytes": plan.lds_bytes,
-                   "coop_budget": plan.coop_budget, "coop_when": args.coop_when, "coop_grace": args.coop_grace,
+                   "coop_budget": plan.coop_budget, "coop_when": args.coop_when,
+                   "coop_grace": args.coop_grace if args.coop_grace else plan.coop_grace,
                    "coop_grid": plan.coop_grid, "coop_block": plan.coop_block},
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -144,13 +144,16 @@ CoopPlan PlanCoop(const LaunchConfig& cfg, int32_t maxOps) {
   return c;
 }
 
-// Default tail grace for a launch shape: the time a tail takes grows with
-// the tiles in flight when the queue runs dry, i.e. with the waves per CU.
-// A single-scene launch with the scene in LDS runs one workgroup per CU:
-// 50 % at 512 lanes (8 waves per CU), 100 % at 1024 (measured best at each
-// on the cheese views and random trees).  Other launches keep 50 %.
-int32_t DefaultCoopGrace(const LaunchShape& shape) {
-  return shape.sceneInLds ? kDefaultCoopGrace * std::max(1, shape.block / 512) : kDefaultCoopGrace;
+// Default tail grace, per mille of the time the tiles took to hand out.
+// After the last tile is taken, the tiles in flight finish in about one
+// tile's time, while handing out took about tiles / waves tile times: the
+// grace is 7900 x waves / tiles per mille, i.e. 790 % x waves / tiles
+// (measured best: 50 % for 512-lane and 100 % for 1024-lane workgroups on
+// the 1920x1080 cheese frame, about 3-4 % for the 256-tree batch).
+int32_t DefaultCoopGraceMille(int64_t waves, int64_t tiles) {
+  if (tiles <= 0) return 1000;
+  const int64_t g = (int64_t{7900} * waves + tiles / 2) / tiles;
+  return static_cast<int32_t>(std::min<int64_t>(std::max<int64_t>(g, 5), 1000000));
 }
 
 void PrepareCoop(const LaunchConfig& cfg, int32_t maxOps, uint2* heavy, const LaunchShape& shape, RenderParams* p) {
@@ -166,7 +169,8 @@ void PrepareCoop(const LaunchConfig& cfg, int32_t maxOps, uint2* heavy, const La
   p->coopQueueCap = c.queueCap;
   p->coopWaves = c.waves;
   p->coopEager = cfg.coopEager ? 1 : 0;
-  p->coopGrace = cfg.coopGrace > 0 ? cfg.coopGrace : (cfg.coopGrace < 0 ? 0 : DefaultCoopGrace(shape));
+  p->coopGrace = cfg.coopGrace > 0 ? cfg.coopGrace * 10
+                                   : (cfg.coopGrace < 0 ? 0 : DefaultCoopGraceMille(p->totalWaves, p->numTiles));
 }
 
 }  // namespace rt
@@ -516,7 +520,11 @@ csg_status csg_render_plan(const csg_scene* s, const csg_render_opts* opts, csg_
   info->coop_budget = c.budget;
   info->coop_grid = static_cast<int32_t>(c.waves);
   info->coop_block = 64;
-  info->coop_grace = c.budget > 0 ? (cfg.coopGrace > 0 ? cfg.coopGrace : (cfg.coopGrace < 0 ? 0 : DefaultCoopGrace(shape)))
+  // the grace of a 1920x1080 frame (240 x 135 tiles)
+  const int64_t hdTiles = 240 * 135;
+  const int64_t waves = int64_t{shape.grid} * (shape.block / 64);
+  info->coop_grace = c.budget > 0 ? (cfg.coopGrace > 0 ? cfg.coopGrace
+                                                       : (cfg.coopGrace < 0 ? 0 : (DefaultCoopGraceMille(waves, hdTiles) + 5) / 10))
                                   : 0;
   info->coop_lds_bytes = static_cast<int64_t>(c.ldsBytes);
   return CSG_OK;

@@ -62,7 +62,7 @@ struct RenderParams {
   // one with a whole wave.  budget <= 0: never defer.
   int32_t budget;
   int32_t coopEager;
-  int32_t coopGrace;          // tail mode: percent of the tile hand-out time to wait after the last tile
+  int32_t coopGrace;          // tail mode: per mille of the tile hand-out time to wait after the last tile
   uint2* heavy;
   uint32_t heavyCap;
   int32_t coopOps;            // operator-state slots per wave (max operators of the launch)
@@ -140,7 +140,6 @@ constexpr int32_t kMaxBatchJobs = 1024;
 // Default per-lane step budget before a ray is handed to the cooperative
 // kernel, and the capacity of the heavy-ray list.
 constexpr int32_t kDefaultBudget = 384;
-constexpr int32_t kDefaultCoopGrace = 50;
 constexpr uint32_t kHeavyCap = 1u << 22;
 // Resume records of handed-over rays: 20 words each (current operator,
 // action, stack slots, tmin, the L/R/held records, the event counts so far),

@@ -600,7 +600,7 @@ __device__ Hit Trace(const SceneView& sv, const RayF& r, uint32_t rootDesc, uint
         }
         if (drainAt != 0) {
           const uint64_t now = wall_clock64();
-          late = (now - drainAt) * 100u >= (drainAt - startAt) * static_cast<uint64_t>(grace);
+          late = (now - drainAt) * 1000u >= (drainAt - startAt) * static_cast<uint64_t>(grace);
         }
       }
     }
