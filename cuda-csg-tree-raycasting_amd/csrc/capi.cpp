@@ -508,7 +508,7 @@ csg_status csg_render_plan(const csg_scene* s, const csg_render_opts* opts, csg_
   hipError_t e = csgray::DeviceShapeLimits(&cus, &ldsLimit);
   if (e != hipSuccess) return FailHip(e, "device query");
   csgray::LaunchShape shape;
-  e = csgray::ChooseShape(s->dev.opDepth, SceneBytes(s), cfg, &shape);
+  e = csgray::ChooseShape(s->dev.opDepth, SceneBytes(s), cfg, &shape, 0, 0, s->dev.nOps);
   if (e != hipSuccess) return FailHip(e, "choosing the launch shape");
   info->block = shape.block;
   info->grid = shape.grid;
@@ -572,7 +572,7 @@ csg_status csg_render_ex(const csg_scene* s, const csg_camera* cam, const csg_li
   p.tileOrder = opts ? opts->tile_order : nullptr;
   p.morton = cfg.morton ? 1 : 0;
   csgray::LaunchShape shape;
-  e = csgray::ChooseShape(s->dev.opDepth, SceneBytes(s), cfg, &shape, 0, p.numTiles);
+  e = csgray::ChooseShape(s->dev.opDepth, SceneBytes(s), cfg, &shape, 0, p.numTiles, s->dev.nOps);
   if (e != hipSuccess) return FailHip(e, "choosing the launch shape");
   csgray::FitGridToTiles(&shape, p.numTiles);
   p.totalWaves = static_cast<uint32_t>(shape.grid) * static_cast<uint32_t>(shape.block / 64);

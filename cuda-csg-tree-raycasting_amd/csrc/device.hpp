@@ -122,9 +122,10 @@ inline void FitGridToTiles(LaunchShape* s, int64_t tiles) {
 // Picks the launch shape on the current device for a launch whose scenes
 // have at most `depth` operator levels; `sceneBytes` > 0 allows staging that
 // many bytes of scene in LDS (single-scene launches only).  `tiles` is the
-// launch's tile count (0: unknown, taken as a large frame).
+// launch's tile count and `ops` its scene's operator count (0: unknown,
+// taken as a large frame and a large tree).
 hipError_t ChooseShape(int32_t depth, size_t sceneBytes, const LaunchConfig& cfg, LaunchShape* shape,
-                       int32_t batchJobs = 0, int64_t tiles = 0);
+                       int32_t batchJobs = 0, int64_t tiles = 0, int32_t ops = 0);
 
 // Compute units and LDS bytes per workgroup of the current device.
 hipError_t DeviceShapeLimits(int* computeUnits, size_t* ldsPerBlock);

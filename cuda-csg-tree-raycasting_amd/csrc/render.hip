@@ -1400,7 +1400,7 @@ hipError_t DeviceShapeLimits(int* computeUnits, size_t* ldsPerBlock) {
 }
 
 hipError_t ChooseShape(int32_t depthIn, size_t sceneBytes, const LaunchConfig& cfg, LaunchShape* shape,
-                       int32_t batchJobs, int64_t tiles) {
+                       int32_t batchJobs, int64_t tiles, int32_t ops) {
   const DeviceLimits* lp = CurrentLimits();
   if (!lp) return hipErrorNoDevice;
   const DeviceLimits& lim = *lp;
@@ -1415,12 +1415,14 @@ hipError_t ChooseShape(int32_t depthIn, size_t sceneBytes, const LaunchConfig& c
   int block = cfg.block > 0 ? cfg.block : 0;
   if (cfg.sceneInLds != 0 && sceneBytes > 0) {
     // The largest workgroup that fits: most waves sharing one scene copy.
-    // 1024 lanes (4 waves per SIMD) hold twice the tiles in flight and take
-    // each about twice as long, so they pay only on frames of at least 5
-    // tiles per wave of that grid: there they measured 4-25 % faster than
-    // 512 on heavy scenes (cheese views, random trees at 1920x1080); below
-    // it up to 25 % slower, and on light scenes a little slower at any size.
-    const bool wide = tiles <= 0 || tiles >= int64_t{5} * cus * 16;
+    // 1024 lanes (4 waves per SIMD) take each tile about twice as long as
+    // 512, which the heavy-ray hand-over hides on scenes with long rays
+    // (measured 2-25 % faster on the 512-sphere cheese views from 640x360 to
+    // 2560x1440) but not on small trees, whose rays are short and never
+    // handed over: there the last tiles' extra time shows (the 7-operator
+    // Wikipedia scene 0.04 ms slower at every size).  So 1024 lanes need a
+    // scene of at least 32 operators and a frame with a tile for every wave.
+    const bool wide = (ops <= 0 || ops >= 32) && (tiles <= 0 || tiles >= int64_t{16} * cus);
     for (int b : {1024, 512, 256}) {
       if (block && b != block) continue;
       if (!block && b == 1024 && !wide) continue;
