@@ -427,7 +427,7 @@ __device__ __forceinline__ Hit Intersect(const SceneView& sv, const RayF& r, uin
 // last tile has reached `grace` percent of the time between the two: the
 // frame's tail.  Lanes of a wave enter together and step together, so
 // `iter` is the same in every active lane and the check (at `budget` steps,
-// then every 128; the global stamps every 256) is wave-uniform.
+// then every 32; the global stamps every 256) is wave-uniform.
 // The state machine's state when a ray is handed over: the operator it
 // stands on, the next action, tmin, the records and the stack depth (the
 // stack itself stays in the caller's slab).
@@ -588,7 +588,7 @@ __device__ Hit Trace(const SceneView& sv, const RayF& r, uint32_t rootDesc, uint
     }
     iter++;
     bool late = false;
-    if (iter >= budget && (iter == budget || (iter & 127) == 0)) {
+    if (iter >= budget && (iter == budget || (iter & 31) == 0)) {
       if (!gstamps) {
         late = true;
       } else {
