@@ -95,10 +95,11 @@ csg_status Stage(csg_batch* b) {
     const int32_t cnt = std::max(0, std::min(n - start, csgray::kMaxBatchJobs));
     if (cnt > ch.cap) return Fail(CSG_E_ARG, "csg_batch: chunk capacity exceeded");
     int64_t base = 0;
-    int32_t depth = 1, maxOps = 1;
+    int32_t depth = 1, maxOps = 1, maxLeaves = 1;
     for (int32_t i = 0; i < cnt; i++) {
       const csg_job& j = b->jobs[order[start + i]];
       maxOps = std::max(maxOps, j.scene->dev.nOps);
+      maxLeaves = std::max(maxLeaves, j.scene->dev.opLeaves);
       const csg_status st = BuildJob(j.scene, &j.camera, &j.light, &j.phong, j.width, j.height, &j.out, &b->opts,
                                      b->device, base, &ch.hostJobs[i]);
       if (st != CSG_OK) return st;
@@ -118,7 +119,7 @@ csg_status Stage(csg_batch* b) {
     if (e != hipSuccess) return FailHip(e, "choosing the launch shape");
     csgray::FitGridToTiles(&ch.shape, base);
     p.totalWaves = static_cast<uint32_t>(ch.shape.grid) * static_cast<uint32_t>(ch.shape.block / 64);
-    PrepareCoop(b->cfg, maxOps, ch.heavy, ch.shape, &p);
+    PrepareCoop(b->cfg, maxOps, maxLeaves, ch.heavy, ch.shape, &p);
   }
   return CSG_OK;
 }

@@ -68,6 +68,7 @@ csg_status Upload(csg_scene* s) {
   s->dev.nOps = f.nOps;
   s->dev.nPrims = f.nPrims;
   s->dev.opDepth = f.opDepth;
+  s->dev.opLeaves = f.opLeaves;
   return CSG_OK;
 }
 
@@ -119,8 +120,9 @@ csg_status HeavyFor(int dev, hipStream_t stream, uint2** out) {
 }
 
 // Budget (0: no deferral), grid and LDS of the cooperative kernel for scenes
-// of at most `maxOps` operators: one-wave workgroups over the scene in global
-// memory, up to 16 per CU as their LDS allows.
+// of at most `maxOps` operators and `maxLeaves` operators without an
+// operator child: one-wave workgroups over the scene in global memory, up to
+// 16 per CU as their LDS allows.
 struct CoopPlan {
   int32_t budget = 0;
   uint32_t waves = 0;
@@ -128,10 +130,10 @@ struct CoopPlan {
   size_t ldsBytes = 0;
 };
 
-CoopPlan PlanCoop(const LaunchConfig& cfg, int32_t maxOps) {
+CoopPlan PlanCoop(const LaunchConfig& cfg, int32_t maxOps, int32_t maxLeaves) {
   CoopPlan c;
   if (cfg.budget < 0 || maxOps <= 0) return c;
-  const size_t wave = CoopLdsBytes(maxOps, &c.queueCap);
+  const size_t wave = CoopLdsBytes(maxOps, maxLeaves, &c.queueCap);
   if (wave == 0) return c;
   int cus = 0;
   size_t ldsPerBlock = 0;
@@ -156,10 +158,11 @@ int32_t DefaultCoopGraceMille(int64_t waves, int64_t tiles) {
   return static_cast<int32_t>(std::min<int64_t>(std::max<int64_t>(g, 5), 1000000));
 }
 
-void PrepareCoop(const LaunchConfig& cfg, int32_t maxOps, uint2* heavy, const LaunchShape& shape, RenderParams* p) {
+void PrepareCoop(const LaunchConfig& cfg, int32_t maxOps, int32_t maxLeaves, uint2* heavy, const LaunchShape& shape,
+                 RenderParams* p) {
   p->budget = 0;
   if (!heavy) return;
-  const CoopPlan c = PlanCoop(cfg, maxOps);
+  const CoopPlan c = PlanCoop(cfg, maxOps, maxLeaves);
   if (c.budget <= 0) return;
   p->budget = c.budget;
   p->heavy = heavy;
@@ -516,7 +519,7 @@ csg_status csg_render_plan(const csg_scene* s, const csg_render_opts* opts, csg_
   info->scene_in_lds = shape.sceneInLds ? 1 : 0;
   info->compute_units = cus;
   info->lds_limit = static_cast<int64_t>(ldsLimit);
-  const CoopPlan c = PlanCoop(cfg, s->dev.nOps);
+  const CoopPlan c = PlanCoop(cfg, s->dev.nOps, s->dev.opLeaves);
   info->coop_budget = c.budget;
   info->coop_grid = static_cast<int32_t>(c.waves);
   info->coop_block = 64;
@@ -584,7 +587,7 @@ csg_status csg_render_ex(const csg_scene* s, const csg_camera* cam, const csg_li
     const csg_status hsd = HeavyFor(dev, hs, &heavy);
     if (hsd != CSG_OK) return hsd;
   }
-  PrepareCoop(cfg, s->dev.nOps, heavy, shape, &p);
+  PrepareCoop(cfg, s->dev.nOps, s->dev.opLeaves, heavy, shape, &p);
   e = csgray::LaunchRender(p, cfg, shape, hs);
   if (e != hipSuccess) return FailHip(e, "render launch");
   return CSG_OK;
@@ -653,9 +656,12 @@ csg_status csg_render_batch(const csg_job* jobs, int32_t njobs, const csg_render
       const csg_status hsd = HeavyFor(dev, hs, &heavy);
       if (hsd != CSG_OK) return hsd;
     }
-    int32_t maxOps = 1;
-    for (int32_t i = 0; i < n; i++) maxOps = std::max(maxOps, jobs[start + i].scene->dev.nOps);
-    PrepareCoop(cfg, maxOps, heavy, shape, &p);
+    int32_t maxOps = 1, maxLeaves = 1;
+    for (int32_t i = 0; i < n; i++) {
+      maxOps = std::max(maxOps, jobs[start + i].scene->dev.nOps);
+      maxLeaves = std::max(maxLeaves, jobs[start + i].scene->dev.opLeaves);
+    }
+    PrepareCoop(cfg, maxOps, maxLeaves, heavy, shape, &p);
     e = csgray::LaunchRender(p, cfg, shape, hs);
     if (e != hipSuccess) return FailHip(e, "batched render launch");
   }

@@ -18,6 +18,7 @@ struct DevScene {
   int32_t nOps;
   int32_t nPrims;
   int32_t opDepth;
+  int32_t opLeaves;  // operators without an operator child: a ray's live cooperative tasks never exceed it
 };
 
 // One render job: a scene seen by one camera at one resolution, with its
@@ -66,7 +67,7 @@ struct RenderParams {
   uint2* heavy;
   uint32_t heavyCap;
   int32_t coopOps;            // operator-state slots per wave (max operators of the launch)
-  uint32_t coopQueueCap;      // task ring entries (power of two >= coopOps)
+  uint32_t coopQueueCap;      // task ring entries (power of two >= the launch's largest opLeaves)
   uint32_t coopWaves;         // waves in the cooperative grid
   // Resume area (kResumeBytes, right after the heavy list): for the first
   // ResumeCap() handed-over rays, the state machine's state at hand-over, so
@@ -154,18 +155,22 @@ constexpr uint32_t kResumeFlag = 1u << 31;
 constexpr size_t kHeavyBytes = sizeof(uint32_t) * 2 * kHeavyCap + kResumeBytes;
 
 // LDS bytes the cooperative kernel needs for scenes of at most `ops`
-// operators (0 when it cannot run: the caller then disables deferral).
-size_t CoopLdsBytes(int32_t ops, uint32_t* queueCap);
+// operators of which at most `leaves` have no operator child (0 when it
+// cannot run: the caller then disables deferral).
+size_t CoopLdsBytes(int32_t ops, int32_t leaves, uint32_t* queueCap);
 // LDS of one wave's cooperative state (one state per operator, the task
-// ring; rounded to 16 B), i.e. of one cooperative workgroup of `p`.
+// ring of 16-bit tasks and their float tmin; rounded to 16 B), i.e. of one
+// cooperative workgroup of `p`.
 #if defined(__HIPCC__)
 #define CSG_HOST_DEVICE __host__ __device__
 #else
 #define CSG_HOST_DEVICE
 #endif
-constexpr size_t kCoopOpStateBytes = 20;
+constexpr size_t kCoopOpStateBytes = 16;
+constexpr size_t kCoopTaskBytes = 6;
 CSG_HOST_DEVICE inline size_t CoopWaveLds(int32_t ops, uint32_t queueCap) {
-  const size_t bytes = static_cast<size_t>(ops) * kCoopOpStateBytes + static_cast<size_t>(queueCap) * 2;
+  const size_t bytes =
+      static_cast<size_t>(ops) * kCoopOpStateBytes + static_cast<size_t>(queueCap) * kCoopTaskBytes;
   return (bytes + 15) & ~static_cast<size_t>(15);
 }
 size_t CoopLaunchLds(const RenderParams& p);

@@ -978,7 +978,13 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(7))) 
 // Every box test, primitive test and decision of the sequential machine
 // happens exactly once, so results and event counts are identical.  At most
 // one task per operator is alive at a time, so task state is indexed by
-// operator.
+// operator.  An ENTER carries its tmin with it (the lane holding the task
+// keeps it in a register, the ring stores it beside the task); a COMPUTE
+// needs none.  Live tasks never include an operator and one of its
+// ancestors (a task's ancestors all wait for its subtree's record), and
+// disjoint subtrees hold disjoint sets of leaves, so a ray never has more
+// live tasks than the tree has operators without an operator child: that
+// count bounds the ring.
 //
 // The count of child records still to arrive (0-2) lives in bits 11-12 of
 // the left record's word `lb`, which no record uses: the right child's
@@ -986,8 +992,7 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(7))) 
 // one count (the word's record bits are clear whenever a left delivery is
 // awaited), both as one atomic, so exactly one of two deliveries sees the
 // count reach zero.  Readers mask the bits off.
-struct OpState {
-  float tmin;
+struct alignas(16) OpState {
   float lt;
   uint32_t lb;  // left record bits | pending count << 11
   float rt;
@@ -1003,7 +1008,7 @@ __device__ __forceinline__ void WaveSync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-static_assert(sizeof(OpState) == kCoopOpStateBytes, "cooperative LDS sizing assumes 20-byte operator states");
+static_assert(sizeof(OpState) == kCoopOpStateBytes, "cooperative LDS sizing assumes 16-byte operator states");
 
 __device__ __forceinline__ uint32_t EnterTask(uint32_t op) { return op << 1; }
 __device__ __forceinline__ uint32_t ComputeTask(uint32_t op) { return (op << 1) | 1u; }
@@ -1030,13 +1035,13 @@ __device__ __forceinline__ uint32_t LanesBelow(uint64_t mask) {
 // ones the sequential machine would have made, so results and counts are
 // unchanged.
 template <bool kFast, bool kCount>
-__device__ Hit CoopTrace(const SceneView& sv, const RayF& r, OpState* st, uint16_t* q, uint32_t qmask,
+__device__ Hit CoopTrace(const SceneView& sv, const RayF& r, OpState* st, uint16_t* q, float* qt, uint32_t qmask,
                          const uint16_t* table, int lane, Counts& n, const uint32_t* rec = nullptr,
                          const uint2* ent = nullptr, const uint32_t* own = nullptr) {
   uint32_t mine = kNone;
+  float mineT = 0.0f;  // the held task's tmin (an ENTER's)
   if (!rec) {
     mine = lane == 0 ? EnterTask(0) : kNone;
-    if (lane == 0) st[0].tmin = 0.0f;
   } else {
     const uint32_t cur = rec[0], action = rec[1];
     const int slots = static_cast<int>(rec[2]);
@@ -1060,9 +1065,9 @@ __device__ Hit CoopTrace(const SceneView& sv, const RayF& r, OpState* st, uint16
       if (a != kNone) {
         if (code == kSlotSaveLft) {
           const uint32_t rc = DescIndex(AsU(sv.ops[4 * a + 3].y));
-          st[rc].tmin = et;
           st[a].lb = 2 * kPendOne;
           mine = EnterTask(rc);
+          mineT = et;
         } else if (code == kSlotLoadLft) {
           st[a].lt = et;
           st[a].lb = eb | kPendOne;
@@ -1093,8 +1098,8 @@ __device__ Hit CoopTrace(const SceneView& sv, const RayF& r, OpState* st, uint16
           const uint32_t old = atomicAdd(&st[cur].lb, (left ? h.bits : 0u) - kPendOne);
           if ((old & kPendMask) == kPendOne) mine = ComputeTask(cur);
         } else {
-          st[DescIndex(d)].tmin = tmin;
           mine = EnterTask(DescIndex(d));
+          mineT = tmin;
         }
       } else {
         // Compute at the current operator; LoadLft / LoadRgh first take the
@@ -1118,10 +1123,15 @@ __device__ Hit CoopTrace(const SceneView& sv, const RayF& r, OpState* st, uint16
     const uint32_t rank = LanesBelow(idle);
     const uint32_t avail = tail - head, nIdle = static_cast<uint32_t>(__popcll(idle));
     const uint32_t take = avail < nIdle ? avail : nIdle;
-    if (mine == kNone && rank < take) mine = q[(head + rank) & qmask];
+    if (mine == kNone && rank < take) {
+      const uint32_t at = (head + rank) & qmask;
+      mine = q[at];
+      mineT = qt[at];
+    }
     head += take;
     if (__ballot(mine != kNone) == 0) break;  // nothing alive and no result: a malformed tree
     uint32_t sp0 = kNone, sp1 = kNone;
+    float spT = 0.0f;  // tmin of the spawned ENTERs (a second one, sp1, is always an ENTER)
     bool fin = false;
     Hit out = MissRecord();
     if (mine != kNone) {
@@ -1133,11 +1143,11 @@ __device__ Hit CoopTrace(const SceneView& sv, const RayF& r, OpState* st, uint16
       const float4 q2 = sv.ops[4 * X + 2];
       const float4 q3 = sv.ops[4 * X + 3];
       const OpState sx = st[X];
-      asm volatile("" ::"v"(q0.x), "v"(q1.x), "v"(q2.x), "v"(q3.x), "v"(sx.tmin), "v"(sx.lt));
+      asm volatile("" ::"v"(q0.x), "v"(q1.x), "v"(q2.x), "v"(q3.x), "v"(sx.lt), "v"(sx.rt));
       const uint32_t lDesc = AsU(q3.x), rDesc = AsU(q3.y);
       if ((mine & 1u) == 0) {
         if (kCount) n.go++;
-        const float tmin = sx.tmin;
+        const float tmin = mineT;
         const bool gL = BoxHit<kFast>(r, q0.x, q0.y, q0.z, q1.x, q1.y, q1.z, tmin);
         const bool gR = BoxHit<kFast>(r, q0.w, q1.w, q2.x, q2.y, q2.z, q2.w, tmin);
         if (kCount) n.box += 2;
@@ -1162,16 +1172,13 @@ __device__ Hit CoopTrace(const SceneView& sv, const RayF& r, OpState* st, uint16
           st[X].rt = R.t;
           st[X].rb = R.bits;
         }
-        if (dL) {
-          st[DescIndex(lDesc)].tmin = tmin;
-          sp0 = EnterTask(DescIndex(lDesc));
-        }
+        if (dL) sp0 = EnterTask(DescIndex(lDesc));
         if (dR) {
-          st[DescIndex(rDesc)].tmin = tmin;
           if (dL) sp1 = EnterTask(DescIndex(rDesc));
           else sp0 = EnterTask(DescIndex(rDesc));
         }
         if (!dL & !dR) sp0 = ComputeTask(X);
+        spT = tmin;
       } else {
         if (kCount) n.comp++;
         const Hit L{sx.lt, sx.lb & ~kPendMask}, R{sx.rt, sx.rb};
@@ -1195,10 +1202,10 @@ __device__ Hit CoopTrace(const SceneView& sv, const RayF& r, OpState* st, uint16
             }
             sp0 = ComputeTask(X);
           } else {
-            st[DescIndex(d)].tmin = tmin;
             // one record to come; a left one arrives into clear bits
             st[X].lb = (loopL ? 0u : L.bits) | kPendOne;
             sp0 = EnterTask(DescIndex(d));
+            spT = tmin;
           }
         } else {
           const uint32_t rb = dec == kOutRetRFlip ? (R.bits ^ (kFlip | kExit | kEnter)) : R.bits;
@@ -1228,8 +1235,13 @@ __device__ Hit CoopTrace(const SceneView& sv, const RayF& r, OpState* st, uint16
     }
     // keep the first spawned task, queue the second
     mine = sp0;
+    mineT = spT;
     const uint64_t bq = __ballot(sp1 != kNone);
-    if (sp1 != kNone) q[(tail + LanesBelow(bq)) & qmask] = static_cast<uint16_t>(sp1);
+    if (sp1 != kNone) {
+      const uint32_t at = (tail + LanesBelow(bq)) & qmask;
+      q[at] = static_cast<uint16_t>(sp1);
+      qt[at] = spT;
+    }
     tail += static_cast<uint32_t>(__popcll(bq));
     WaveSync();
   }
@@ -1248,7 +1260,8 @@ __global__ void __launch_bounds__(64) CoopKernel(RenderParams p) {
   const int lane = threadIdx.x;
   if (lane < kDecisionEntries) table[lane] = kDecisions.v[lane];
   OpState* st = reinterpret_cast<OpState*>(ldsBase);
-  uint16_t* q = reinterpret_cast<uint16_t*>(st + p.coopOps);
+  float* qt = reinterpret_cast<float*>(st + p.coopOps);
+  uint16_t* q = reinterpret_cast<uint16_t*>(qt + p.coopQueueCap);
   const uint32_t qmask = p.coopQueueCap - 1;
   __syncthreads();
   const uint32_t listed = __hip_atomic_load(p.queue + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -1296,7 +1309,7 @@ __global__ void __launch_bounds__(64) CoopKernel(RenderParams p) {
       ent = ResumeEntries(p, rcap) + static_cast<size_t>(k) * cap;
       own = ResumeOwners(p, rcap, cap) + static_cast<size_t>(k) * cap;
     }
-    const Hit h = CoopTrace<kFast, kCount>(sv, r, st, q, qmask, table, lane, n, rec, ent, own);
+    const Hit h = CoopTrace<kFast, kCount>(sv, r, st, q, qt, qmask, table, lane, n, rec, ent, own);
     if (kCount && jb.counters) {
       int32_t v[6] = {n.go, n.comp, n.box, n.sph, n.cyl, n.cub};
       for (int c = 0; c < 6; c++) {
@@ -1380,10 +1393,12 @@ int32_t MaxSupportedDepth() { return kMaxDepth; }
 
 size_t CoopLaunchLds(const RenderParams& p) { return CoopWaveLds(p.coopOps, p.coopQueueCap); }
 
-size_t CoopLdsBytes(int32_t ops, uint32_t* queueCap) {
+size_t CoopLdsBytes(int32_t ops, int32_t leaves, uint32_t* queueCap) {
   if (ops > 32767) return 0;  // ring entries hold op << 1 | kind in 16 bits
+  // live tasks <= operators without an operator child (at least one; at most all)
+  const int32_t live = leaves < 1 ? 1 : (leaves > ops ? ops : leaves);
   uint32_t cap = 1;
-  while (cap < static_cast<uint32_t>(ops > 0 ? ops : 1)) cap <<= 1;
+  while (cap < static_cast<uint32_t>(live)) cap <<= 1;
   *queueCap = cap;
   const size_t bytes = CoopWaveLds(ops, cap);
   const DeviceLimits* l = CurrentLimits();

@@ -262,6 +262,7 @@ FlatScene Flatten(const CSGTree& tree) {
     const bool isRight = n.parent >= 0 && tree.nodes[n.parent].right == i;
     const uint32_t typeWord = static_cast<uint32_t>(n.type) | (isRight ? 0x100u : 0u);
     q[3] = F4{bits(desc(n.left)), bits(desc(n.right)), bits(typeWord), bits(parent)};
+    if (tree.nodes[n.left].primitiveIdx != -1 && tree.nodes[n.right].primitiveIdx != -1) f.opLeaves++;
   }
   f.primA.resize(f.nPrims);
   f.primB.resize(f.nPrims);

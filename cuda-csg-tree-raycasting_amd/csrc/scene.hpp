@@ -87,6 +87,7 @@ struct FlatScene {
   std::vector<F4> ops, primA, primB, primC;
   uint32_t rootDesc = 0;
   int32_t nNodes = 0, nOps = 0, nPrims = 0, opDepth = 0;
+  int32_t opLeaves = 0;  // operators without an operator child
 };
 
 constexpr int32_t kMaxNodes = 65535;       // 16-bit node links

@@ -177,9 +177,13 @@ void Dispatch(csg_server* s, Slot& sl) {
     p.queue = queue;
     csgray::FitGridToTiles(&shape, base);
     p.totalWaves = static_cast<uint32_t>(shape.grid) * static_cast<uint32_t>(shape.block / 64);
-    int32_t maxOps = 1;
-    for (int64_t tk : take) maxOps = std::max(maxOps, s->scenes[s->reqs[tk].req.scene_id]->dev.nOps);
-    PrepareCoop(s->launch, maxOps, heavy, shape, &p);
+    int32_t maxOps = 1, maxLeaves = 1;
+    for (int64_t tk : take) {
+      const csgray::DevScene& d = s->scenes[s->reqs[tk].req.scene_id]->dev;
+      maxOps = std::max(maxOps, d.nOps);
+      maxLeaves = std::max(maxLeaves, d.opLeaves);
+    }
+    PrepareCoop(s->launch, maxOps, maxLeaves, heavy, shape, &p);
     if (csgray::LaunchRender(p, s->launch, shape, sl.stream) != hipSuccess) st = CSG_E_HIP;
   }
   if (st == CSG_OK && hipEventRecord(sl.done, sl.stream) != hipSuccess) st = CSG_E_HIP;

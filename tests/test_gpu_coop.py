@@ -186,10 +186,12 @@ def test_plan_reports_cooperative_kernel(gpu):
     s = C.Scene(S.cheese(512, seed=512))
     p = s.plan()
     print(f"\ncoop budget={p.coop_budget} grid={p.coop_grid} lds={p.coop_lds_bytes}")
-    # one wave's LDS: 20 B per operator (512: 511 unions and the difference) and a 2-byte entry per
-    # task-ring slot (512), 16-byte rounded
+    # one wave's LDS: 16 B per operator (512: 511 unions and the difference) and a 6-byte entry (task,
+    # tmin) per task-ring slot, as many as the operators without an operator child (256), 16-byte rounded
     assert p.coop_budget > 0 and p.coop_grid >= p.compute_units
-    assert p.coop_lds_bytes == (512 * 20 + 512 * 2 + 15) // 16 * 16
+    assert p.coop_lds_bytes == (512 * 16 + 256 * 6 + 15) // 16 * 16
+    # 16 one-wave groups per CU fit
+    assert p.coop_grid == 16 * p.compute_units
     assert s.plan(coop=-1).coop_budget == 0
     assert s.plan(coop=7).coop_budget == 7
     s.close()
@@ -211,3 +213,30 @@ def test_default_tail_hand_over_engages_on_the_benchmark_frame(gpu):
     print(f"rays handed over: {moved} (off: {off})")
     assert moved > 1000 and off == 0
     _same(a, b)
+
+
+def test_every_bottom_operator_live_at_once(gpu):
+    """1024 nearly concentric spheres under a balanced Union: a ray through the
+    cluster hits every bounding box, so the 512 bottom operators (those without
+    an operator child, which size the task ring) all have an ENTER live at
+    once, far more than the wave's 64 lanes hold; every ray goes cooperative
+    after one step and must match the oracle bit for bit."""
+    print("\n[case] 1024 concentric spheres, exact, budget 1 eager", flush=True)
+    rng = np.random.default_rng(77)
+    c = rng.uniform(-0.01, 0.01, size=(1024, 3))
+    r = rng.uniform(0.3, 1.0, size=1024)
+    text = "\n".join(S.balanced("Union", [S.sphere(c[i], "F5F500", r[i]) for i in range(1024)])) + "\n"
+    sname, cam, light, phong, cam6, l3, ph8 = setups()[1]
+    s = C.Scene(text)
+    p = s.plan()
+    assert p.coop_lds_bytes == (1023 * 16 + 512 * 6 + 15) // 16 * 16
+    w, h = 96, 54
+    g = _np(s.render(cam, light, phong, w, h, mode=C.MODE_EXACT, counters=True, coop=1, coop_when=1))
+    s.close()
+    o = _oracle_exact(text, cam6, l3, ph8, w, h)
+    # rays through the cluster test every operator's children
+    assert int((o["counters"][..., 2] >= 2 * 1023).sum()) > 100
+    assert np.array_equal(g["prim_id"], o["prim_id"])
+    assert np.array_equal(g["flags"], o["flags"])
+    assert np.array_equal(g["depth"].view(np.uint32), o["depth"].view(np.uint32))
+    assert np.array_equal(g["counters"], o["counters"])
