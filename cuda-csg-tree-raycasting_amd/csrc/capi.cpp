@@ -149,12 +149,14 @@ CoopPlan PlanCoop(const LaunchConfig& cfg, int32_t maxOps, int32_t maxLeaves) {
 // Default tail grace, per mille of the time the tiles took to hand out.
 // After the last tile is taken, the tiles in flight finish in about one
 // tile's time, while handing out took about tiles / waves tile times: the
-// grace is 7900 x waves / tiles per mille, i.e. 790 % x waves / tiles
-// (measured best: 50 % for 512-lane and 100 % for 1024-lane workgroups on
-// the 1920x1080 cheese frame, about 3-4 % for the 256-tree batch).
-int32_t DefaultCoopGraceMille(int64_t waves, int64_t tiles) {
+// grace is k x waves / tiles per mille, k = 9500 for single-scene launches
+// (950 % x waves / tiles) and 5500 for batched ones (550 %).  Measured
+// best: 120-130 % for 1024-lane workgroups on the 1920x1080 cheese frame
+// (k = 9500 gives 120 %), about 3 % for the 256-tree batch (k = 5500).
+int32_t DefaultCoopGraceMille(int64_t waves, int64_t tiles, bool batched) {
   if (tiles <= 0) return 1000;
-  const int64_t g = (int64_t{7900} * waves + tiles / 2) / tiles;
+  const int64_t k = batched ? 5500 : 9500;
+  const int64_t g = (k * waves + tiles / 2) / tiles;
   return static_cast<int32_t>(std::min<int64_t>(std::max<int64_t>(g, 5), 1000000));
 }
 
@@ -173,7 +175,7 @@ void PrepareCoop(const LaunchConfig& cfg, int32_t maxOps, int32_t maxLeaves, uin
   p->coopWaves = c.waves;
   p->coopEager = cfg.coopEager ? 1 : 0;
   p->coopGrace = cfg.coopGrace > 0 ? cfg.coopGrace * 10
-                                   : (cfg.coopGrace < 0 ? 0 : DefaultCoopGraceMille(p->totalWaves, p->numTiles));
+                                   : (cfg.coopGrace < 0 ? 0 : DefaultCoopGraceMille(p->totalWaves, p->numTiles, p->numJobs > 0));
 }
 
 }  // namespace rt
@@ -527,7 +529,7 @@ csg_status csg_render_plan(const csg_scene* s, const csg_render_opts* opts, csg_
   const int64_t hdTiles = 240 * 135;
   const int64_t waves = int64_t{shape.grid} * (shape.block / 64);
   info->coop_grace = c.budget > 0 ? (cfg.coopGrace > 0 ? cfg.coopGrace
-                                                       : (cfg.coopGrace < 0 ? 0 : (DefaultCoopGraceMille(waves, hdTiles) + 5) / 10))
+                                                       : (cfg.coopGrace < 0 ? 0 : (DefaultCoopGraceMille(waves, hdTiles, false) + 5) / 10))
                                   : 0;
   info->coop_lds_bytes = static_cast<int64_t>(c.ldsBytes);
   return CSG_OK;

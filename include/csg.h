@@ -119,10 +119,10 @@ typedef struct csg_render_opts {
   int32_t coop_when;    /* 0: hand over only in the frame's tail, once every tile has been taken and a
                            further coop_grace percent of the time that took has passed;
                            1: as soon as the budget is spent */
-  int32_t coop_grace;   /* 0 = default: 790 x waves / tiles percent (the tiles in flight when the queue runs
-                           dry finish in about a tile's time; handing out took tiles / waves tile times):
-                           100 for a 1024-lane launch of 1920x1080, about 4 for the 256-tree batch;
-                           < 0 = none; else percent */
+  int32_t coop_grace;   /* 0 = default: 950 x waves / tiles percent for one scene, 550 x for a batch (the
+                           tiles in flight when the queue runs dry finish in about a tile's time; handing
+                           out took tiles / waves tile times): 120 for a 1024-lane launch of 1920x1080,
+                           about 3 for the 256-tree batch; < 0 = none; else percent */
 } csg_render_opts;
 
 /* --- scene (replaces CSGTree::Parse + the upload in Raycaster::ChangeSize) --- */
@@ -258,7 +258,7 @@ typedef struct csg_launch_info {
   int32_t coop_grid;      /* workgroups of the cooperative kernel (one ray in flight each) */
   int32_t coop_block;     /* its threads per workgroup (64) */
   int32_t coop_grace;    /* tail grace (percent) a 1920x1080 frame would get (the default is
-                            790 x waves / tiles percent) */
+                            950 x waves / tiles percent) */
   int64_t coop_lds_bytes; /* its dynamic LDS per workgroup */
 } csg_launch_info;
 csg_status csg_render_plan(const csg_scene* scene, const csg_render_opts* opts, csg_launch_info* info);

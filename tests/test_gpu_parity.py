@@ -159,11 +159,11 @@ def test_launch_plan(gpu):
     lds = s.plan(placement=C.PLACE_LDS)
     assert lds.scene_in_lds == 1 and lds.lds_bytes <= lds.lds_limit
     # the largest workgroup whose stacks and scene copy fit, and the default tail grace of a 1920x1080
-    # frame (240 x 135 tiles): 790 % x waves / tiles
-    assert auto.scene_in_lds == 1 and auto.block == 1024 and auto.coop_grace == 100
+    # frame (240 x 135 tiles): 950 % x waves / tiles
+    assert auto.scene_in_lds == 1 and auto.block == 1024 and auto.coop_grace == 120
     for plan in (auto, s.plan(block=512), g):
-        assert plan.coop_grace == round(790 * plan.grid * (plan.block // 64) / (240 * 135))
-    assert s.plan(block=512).coop_grace == 50
+        assert plan.coop_grace == round(950 * plan.grid * (plan.block // 64) / (240 * 135))
+    assert s.plan(block=512).coop_grace == 60
     assert s.plan(coop_grace=30).coop_grace == 30 and s.plan(coop=-1).coop_grace == 0
 
 
