@@ -51,8 +51,8 @@ namespace {
 csg_status AllocChunk(Chunk& c, int32_t cap) {
   hipError_t e = hipMalloc(&c.devJobs, sizeof(csgray::Job) * cap);
   if (e == hipSuccess) e = hipHostMalloc(&c.hostJobs, sizeof(csgray::Job) * cap, hipHostMallocDefault);
-  if (e == hipSuccess) e = hipMalloc(&c.queue, 256);
-  if (e == hipSuccess) e = hipMemset(c.queue, 0, 256);
+  if (e == hipSuccess) e = hipMalloc(&c.queue, csgray::kQueueBytes);
+  if (e == hipSuccess) e = hipMemset(c.queue, 0, csgray::kQueueBytes);
   if (e == hipSuccess) e = hipMalloc(&c.heavy, csgray::kHeavyBytes);
   if (e != hipSuccess) return e == hipErrorOutOfMemory ? Fail(CSG_E_OOM, "batch allocation failed") : FailHip(e, "batch allocation");
   c.cap = cap;

@@ -72,8 +72,9 @@ csg_status Upload(csg_scene* s) {
   return CSG_OK;
 }
 
-// The render kernel's tile queue: two words per (device, stream), zero
-// between launches (the last wave of each launch zeroes them).  Renders on
+// The render kernels' work queues (tile and heavy-ray counters, time stamps;
+// kQueueBytes) per (device, stream), zero between launches (the last
+// workgroup of each launch zeroes them).  Renders on
 // one stream are ordered, so they can share it; separate streams get separate
 // queues.  Queues live until the process ends.
 std::mutex g_queueMutex;
@@ -87,10 +88,10 @@ csg_status QueueFor(int dev, hipStream_t stream, uint32_t** out) {
     return CSG_OK;
   }
   void* q = nullptr;
-  hipError_t e = hipMalloc(&q, 256);
+  hipError_t e = hipMalloc(&q, csgray::kQueueBytes);
   if (e != hipSuccess) return e == hipErrorOutOfMemory ? Fail(CSG_E_OOM, "queue allocation failed")
                                                        : FailHip(e, "hipMalloc");
-  e = hipMemsetAsync(q, 0, 256, stream);
+  e = hipMemsetAsync(q, 0, csgray::kQueueBytes, stream);
   if (e != hipSuccess) {
     (void)hipFree(q);
     return FailHip(e, "hipMemsetAsync");

@@ -53,7 +53,7 @@ struct RenderParams {
   int32_t numJobs;            // 0 = plain
   int32_t numTiles;           // queue length (all jobs)
   int32_t stackSlots;         // LDS stack slots per lane (max operator depth of the launch)
-  uint32_t totalWaves;        // waves in the grid (queue reset bookkeeping)
+  uint32_t totalWaves;        // waves in the grid (sets the default tail grace)
   uint32_t* queue;            // [0] next tile, [1] waves finished; zero between launches
   const uint32_t* tileOrder;  // optional: order in which queue entries map to tiles (plain)
   int32_t morton;             // 1: tiles of a job are visited in Morton order
@@ -74,9 +74,17 @@ struct RenderParams {
   // the cooperative kernel continues the ray instead of restarting it.
   uint32_t* resume;
 };
-// queue words: [0] next tile, [1] tile waves done, [2] heavy rays, [3] next heavy ray, [4] coop waves done,
-// [6..7] heavy rays handed over on this stream so far (64-bit); on their own cache line, [32..33] time
-// the first tile was taken, [34..35] time the tiles ran out (64-bit wall clock | 1, 0 = not yet)
+// queue words: [1] render workgroups done, [2] heavy rays, [4] coop waves done, [6..7] heavy rays handed
+// over on this stream so far (64-bit); on their own cache line, [32..33] time the first tile was taken,
+// [34..35] time the tiles ran out (64-bit wall clock | 1, 0 = not yet).  Tiles and heavy rays are taken
+// from kQueueStripes counters each, 256 B apart: stripe s hands out items s, s + kQueueStripes, ...
+// (counter = items of the stripe taken so far), so the waves' atomics spread over many addresses
+// instead of queueing on one.
+constexpr uint32_t kQueueStripes = 16;
+constexpr uint32_t kStripeWords = 64;
+constexpr uint32_t kTileStripeWord = 64;
+constexpr uint32_t kHeavyStripeWord = kTileStripeWord + kQueueStripes * kStripeWords;
+constexpr size_t kQueueBytes = sizeof(uint32_t) * (kHeavyStripeWord + kQueueStripes * kStripeWords);
 
 // Jitter of sample s of pixel p: a 32-bit integer hash of (seed, p, s)
 // mapped to [0, 1) with 24 bits; identical on host and device.

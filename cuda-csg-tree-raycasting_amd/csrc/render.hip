@@ -886,21 +886,41 @@ __device__ __forceinline__ void RenderBody(const RenderParams& p) {
 
   const int lane = tid & 63;
   uint64_t* stamps = reinterpret_cast<uint64_t*>(p.queue + 32);
+  uint32_t* tileCtr = p.queue + kTileStripeWord;
+  // this wave's home stripe; once it is empty the wave takes tiles from any
+  // stripe that has some left
+  uint32_t stripe = __builtin_amdgcn_readfirstlane((blockIdx.x * (blockDim.x >> 6) + (tid >> 6)) % kQueueStripes);
   for (;;) {
-    uint32_t q = 0;
-    if (lane == 0) q = atomicAdd(p.queue, 1u);
-    q = __builtin_amdgcn_readfirstlane(q);
-    // time stamps of the first tile and of the first pop past the last one
-    // (every lane stores the same value: no branch on the lane index here)
+    uint32_t j = 0;
+    if (lane == 0) j = atomicAdd(tileCtr + stripe * kStripeWords, 1u);
+    j = __builtin_amdgcn_readfirstlane(j);
+    const uint32_t q = stripe + kQueueStripes * j;
+    // time stamp of the first tile (every lane stores the same value: no
+    // branch on the lane index here)
     if (q == 0) __hip_atomic_store(stamps, wall_clock64() | 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (q >= static_cast<uint32_t>(p.numTiles)) {
-      uint64_t drainAt = 0;
-      if (q == static_cast<uint32_t>(p.numTiles)) {
-        drainAt = wall_clock64() | 1u;
-        __hip_atomic_store(stamps + 1, drainAt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      } else {
-        drainAt = __hip_atomic_load(stamps + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      // this stripe is empty: lane i reads stripe i's counter, and the wave
+      // moves to the next stripe (after this one) with tiles left
+      const uint32_t c = lane < static_cast<int>(kQueueStripes)
+                             ? __hip_atomic_load(tileCtr + lane * kStripeWords, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT)
+                             : 0u;
+      const bool left = lane < static_cast<int>(kQueueStripes) &&
+                        static_cast<uint64_t>(lane) + uint64_t{kQueueStripes} * c < static_cast<uint64_t>(p.numTiles);
+      const uint64_t m = __ballot(left);
+      if (m != 0) {
+        const uint64_t after = m & (~uint64_t{0} << (stripe + 1));
+        stripe = static_cast<uint32_t>(__ffsll(static_cast<unsigned long long>(after != 0 ? after : m)) - 1);
+        continue;
       }
+      // every tile has been taken: the first wave to see it stamps the time
+      // (one lane; a plain read first, so later waves add no atomic)
+      if (lane == 0 && __hip_atomic_load(stamps + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) {
+        uint64_t unset = 0;
+        __hip_atomic_compare_exchange_strong(stamps + 1, &unset, wall_clock64() | 1u, __ATOMIC_RELAXED,
+                                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+      const uint64_t drainAt = __hip_atomic_load(stamps + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       // tell the waves of this workgroup that are still tracing
       if (drainAt != 0 && ldsStamps[1] == 0) {
         ldsStamps[0] = __hip_atomic_load(stamps, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -916,26 +936,28 @@ __device__ __forceinline__ void RenderBody(const RenderParams& p) {
         if (static_cast<uint32_t>(jobBase[mid]) <= q) lo = mid;
         else hi = mid - 1;
       }
-      const int j = __builtin_amdgcn_readfirstlane(lo);
-      const Job& jb = p.jobs[j];
+      const int jj = __builtin_amdgcn_readfirstlane(lo);
+      const Job& jb = p.jobs[jj];
       const SceneView jsv{jb.sc.ops, jb.sc.primA, jb.sc.primB};
       RenderTile<kFast, kCount>(jb, jsv, q - static_cast<uint32_t>(jb.tileBase), p.morton != 0, slab + tid,
-                                blockDim.x, cap, table, lane, p, static_cast<uint32_t>(j), ldsStamps);
+                                blockDim.x, cap, table, lane, p, static_cast<uint32_t>(jj), ldsStamps);
     } else {
       const uint32_t t = p.tileOrder ? p.tileOrder[q] : q;
       RenderTile<kFast, kCount>(p.job, sv, t, p.morton != 0, slab + tid, blockDim.x, cap, table, lane, p, 0u,
                                 ldsStamps);
     }
   }
-  // The last wave out returns the queue and the time stamps to zero for the
-  // next launch.
-  if (lane == 0) {
+  // The last workgroup out returns the tile counters and the time stamps to
+  // zero for the next launch (one atomic per workgroup, not per wave).
+  __syncthreads();
+  if (tid == 0) {
     const uint32_t finished = atomicAdd(p.queue + 1, 1u);
-    if (finished == p.totalWaves - 1) {
+    if (finished == gridDim.x - 1) {
+      for (uint32_t i = 0; i < kQueueStripes; i++)
+        __hip_atomic_store(tileCtr + i * kStripeWords, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       __hip_atomic_store(stamps, uint64_t{0}, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       __hip_atomic_store(stamps + 1, uint64_t{0}, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      atomicExch(p.queue, 0u);
-      atomicExch(p.queue + 1, 0u);
+      __hip_atomic_store(p.queue + 1, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
   }
 }
@@ -1276,12 +1298,27 @@ __global__ void __launch_bounds__(64) CoopKernel(RenderParams p) {
   // back to the readfirstlane without lane 0 (they would then read k = 0
   // forever).  So every lane stores the ray's results (the same bytes), and a
   // wave barrier keeps the fetch block from being duplicated.
+  uint32_t* heavyCtr = p.queue + kHeavyStripeWord;
+  uint32_t stripe = blockIdx.x % kQueueStripes;  // home stripe; then any stripe with rays left
   for (;;) {
     __builtin_amdgcn_wave_barrier();
-    uint32_t k = 0;
-    if (lane == 0) k = atomicAdd(p.queue + 3, 1u);
-    k = __builtin_amdgcn_readfirstlane(k);
-    if (k >= count) break;
+    uint32_t j = 0;
+    if (lane == 0) j = atomicAdd(heavyCtr + stripe * kStripeWords, 1u);
+    j = __builtin_amdgcn_readfirstlane(j);
+    const uint32_t k = stripe + kQueueStripes * j;
+    if (k >= count) {
+      const uint32_t c = lane < static_cast<int>(kQueueStripes)
+                             ? __hip_atomic_load(heavyCtr + lane * kStripeWords, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT)
+                             : 0u;
+      const bool left = lane < static_cast<int>(kQueueStripes) &&
+                        static_cast<uint64_t>(lane) + uint64_t{kQueueStripes} * c < static_cast<uint64_t>(count);
+      const uint64_t m = __ballot(left);
+      if (m == 0) break;
+      const uint64_t after = m & (~uint64_t{0} << (stripe + 1));
+      stripe = static_cast<uint32_t>(__ffsll(static_cast<unsigned long long>(after != 0 ? after : m)) - 1);
+      continue;
+    }
     const uint2 e = p.heavy[k];
     const uint32_t jobWord = __builtin_amdgcn_readfirstlane(e.x);
     const Job& jb = kBatch ? p.jobs[jobWord & ~kResumeFlag] : p.job;
@@ -1326,9 +1363,10 @@ __global__ void __launch_bounds__(64) CoopKernel(RenderParams p) {
     const uint32_t finished = atomicAdd(p.queue + 4, 1u);
     if (finished == active - 1) {
       atomicAdd(reinterpret_cast<unsigned long long*>(p.queue + 6), static_cast<unsigned long long>(count));
-      atomicExch(p.queue + 2, 0u);
-      atomicExch(p.queue + 3, 0u);
-      atomicExch(p.queue + 4, 0u);
+      for (uint32_t i = 0; i < kQueueStripes; i++)
+        __hip_atomic_store(heavyCtr + i * kStripeWords, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(p.queue + 2, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(p.queue + 4, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
   }
 }
