@@ -80,7 +80,7 @@ struct RenderParams {
 // from kQueueStripes counters each, 256 B apart: stripe s hands out items s, s + kQueueStripes, ...
 // (counter = items of the stripe taken so far), so the waves' atomics spread over many addresses
 // instead of queueing on one.
-constexpr uint32_t kQueueStripes = 16;
+constexpr uint32_t kQueueStripes = 32;
 constexpr uint32_t kStripeWords = 64;
 constexpr uint32_t kTileStripeWord = 64;
 constexpr uint32_t kHeavyStripeWord = kTileStripeWord + kQueueStripes * kStripeWords;

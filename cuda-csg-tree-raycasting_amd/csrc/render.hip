@@ -854,6 +854,14 @@ __device__ __forceinline__ void RenderTile(const Job& jb, const SceneView& sv, u
 // Persistent render kernel.  LDS: [stack slab: stackSlots x blockDim x 8 B]
 // then, when kLds, the scene copy (plain launches), or, when batched, the
 // jobs' first queue indices.
+// The first stripe after `stripe` (wrapping around) among the set bits of
+// `m` (m != 0, bit i = stripe i has work left).
+__device__ __forceinline__ uint32_t NextStripe(uint64_t m, uint32_t stripe) {
+  static_assert(kQueueStripes <= 64, "one lane per stripe");
+  const uint64_t after = stripe + 1 < 64 ? m & (~uint64_t{0} << (stripe + 1)) : 0;
+  return static_cast<uint32_t>(__ffsll(static_cast<unsigned long long>(after != 0 ? after : m)) - 1);
+}
+
 template <bool kFast, bool kCount, bool kLds, bool kBatch>
 __device__ __forceinline__ void RenderBody(const RenderParams& p) {
   extern __shared__ uint4 ldsBase[];
@@ -909,8 +917,7 @@ __device__ __forceinline__ void RenderBody(const RenderParams& p) {
                         static_cast<uint64_t>(lane) + uint64_t{kQueueStripes} * c < static_cast<uint64_t>(p.numTiles);
       const uint64_t m = __ballot(left);
       if (m != 0) {
-        const uint64_t after = m & (~uint64_t{0} << (stripe + 1));
-        stripe = static_cast<uint32_t>(__ffsll(static_cast<unsigned long long>(after != 0 ? after : m)) - 1);
+        stripe = NextStripe(m, stripe);
         continue;
       }
       // every tile has been taken: the first wave to see it stamps the time
@@ -1315,8 +1322,7 @@ __global__ void __launch_bounds__(64) CoopKernel(RenderParams p) {
                         static_cast<uint64_t>(lane) + uint64_t{kQueueStripes} * c < static_cast<uint64_t>(count);
       const uint64_t m = __ballot(left);
       if (m == 0) break;
-      const uint64_t after = m & (~uint64_t{0} << (stripe + 1));
-      stripe = static_cast<uint32_t>(__ffsll(static_cast<unsigned long long>(after != 0 ? after : m)) - 1);
+      stripe = NextStripe(m, stripe);
       continue;
     }
     const uint2 e = p.heavy[k];
