@@ -187,6 +187,32 @@ def test_queue_reuse_across_sizes_scenes_and_streams(gpu):
                 assert torch.equal(a[k], b[k]) and torch.equal(a[k], c[k]), k
 
 
+@pytest.mark.parametrize("block", [256, 1024])
+def test_launches_smaller_than_the_queue_stripes(gpu, block):
+    """Frames of 1 to 6 tiles (fewer tiles, and with 256-lane workgroups fewer
+    waves, than the work queue has stripes), each rendered twice in a row
+    (the counters must return to zero after every launch), with and without
+    every ray going through the cooperative kernel: bit-identical to the
+    oracle."""
+    print(f"\n[case] tiny frames, block={block}", flush=True)
+    sname, cam, light, phong, cam6, l3, ph8 = setups()[1]
+    text = scene_text("testCheese256.txt")
+    s = C.Scene(text)
+    o = O.OracleScene(text)
+    for w, h in ((2, 2), (5, 3), (8, 8), (17, 9), (9, 17)):
+        want = o.render(cam6, l3, ph8, w, h, bits=32, counters=True)
+        for coop, when in ((0, 0), (1, 1), (1, 0)):
+            for _ in range(2):
+                g = s.render(cam, light, phong, w, h, mode=C.MODE_EXACT, counters=True, block=block,
+                             coop=coop, coop_when=when)
+                g = {k: v.cpu().numpy() for k, v in g.items()}
+                assert np.array_equal(g["prim_id"], want["prim_id"]), (w, h, coop, when)
+                assert np.array_equal(g["depth"].view(np.uint32), want["depth"].view(np.uint32)), (w, h)
+                assert np.array_equal(g["flags"], want["flags"]), (w, h)
+                assert np.array_equal(g["counters"], want["counters"]), (w, h)
+    s.close()
+
+
 def test_render_is_deterministic_and_stream_ordered(gpu):
     sname, cam, light, phong, cam6, l3, ph8 = setups()[1]
     s = C.Scene(scene_text("testCheese256.txt"))
