@@ -1475,13 +1475,16 @@ hipError_t ChooseShape(int32_t depthIn, size_t sceneBytes, const LaunchConfig& c
   if (cfg.sceneInLds != 0 && sceneBytes > 0) {
     // The largest workgroup that fits: most waves sharing one scene copy.
     // 1024 lanes (4 waves per SIMD) take each tile about twice as long as
-    // 512, which the heavy-ray hand-over hides on scenes with long rays
-    // (measured 2-25 % faster on the 512-sphere cheese views from 640x360 to
-    // 2560x1440) but not on small trees, whose rays are short and never
-    // handed over: there the last tiles' extra time shows (the 7-operator
-    // Wikipedia scene 0.04 ms slower at every size).  So 1024 lanes need a
-    // scene of at least 32 operators and a frame with a tile for every wave.
-    const bool wide = (ops <= 0 || ops >= 32) && (tiles <= 0 || tiles >= int64_t{16} * cus);
+    // 512, which the heavy-ray hand-over hides on scenes with long rays but
+    // not on smaller trees, whose rays are short and rarely handed over.
+    // Measured on cheese scenes at 1920x1080 and 3840x2160: 512 lanes are
+    // faster below 256 operators (up to 40 %), level at 256, and 1024 faster
+    // from 384 up (13-30 %); the 7-operator Wikipedia scene is 2x slower with
+    // 1024.  So 1024 lanes need a scene of at least 256 operators and a
+    // frame of at least 4 tiles for every wave of that grid (the 512-sphere
+    // cheese at 1280x720, 3.5 tiles a wave: 1.10 ms with 512 lanes, 1.23
+    // with 1024; at 1920x1080, 7.9 a wave: 1.77 against 1.54).
+    const bool wide = (ops <= 0 || ops >= 256) && (tiles <= 0 || tiles >= int64_t{64} * cus);
     for (int b : {1024, 512, 256}) {
       if (block && b != block) continue;
       if (!block && b == 1024 && !wide) continue;
