@@ -1481,10 +1481,12 @@ hipError_t ChooseShape(int32_t depthIn, size_t sceneBytes, const LaunchConfig& c
     // faster below 256 operators (up to 40 %), level at 256, and 1024 faster
     // from 384 up (13-30 %); the 7-operator Wikipedia scene is 2x slower with
     // 1024.  So 1024 lanes need a scene of at least 256 operators and a
-    // frame of at least 4 tiles for every wave of that grid (the 512-sphere
-    // cheese at 1280x720, 3.5 tiles a wave: 1.10 ms with 512 lanes, 1.23
-    // with 1024; at 1920x1080, 7.9 a wave: 1.77 against 1.54).
-    const bool wide = (ops <= 0 || ops >= 256) && (tiles <= 0 || tiles >= int64_t{64} * cus);
+    // frame with a tile for every wave of that grid.  On the 256 random
+    // trees of the batched mix (128^2 to 1024^2, each its own launch) that
+    // rule costs 222 ms against 220 for the best such rule (32 operators,
+    // a tile a wave); requiring 4 tiles a wave, 230 ms: heavy trees at
+    // 512x512 hand their tails over only with 1024 lanes (1.5-1.9x faster).
+    const bool wide = (ops <= 0 || ops >= 256) && (tiles <= 0 || tiles >= int64_t{16} * cus);
     for (int b : {1024, 512, 256}) {
       if (block && b != block) continue;
       if (!block && b == 1024 && !wide) continue;

@@ -246,7 +246,7 @@ csg_status csg_batch_free(csg_batch* batch);
  * current device, for a large frame (no reference counterpart; for tests and
  * tuning).  A single-scene launch with the scene in LDS takes 1024-lane
  * workgroups when they fit, the scene has at least 256 operators and the frame
- * has four 8x8 tiles for every wave of that grid; otherwise 512 lanes. */
+ * has an 8x8 tile for every wave of that grid; otherwise 512 lanes. */
 typedef struct csg_launch_info {
   int32_t block;          /* threads per workgroup */
   int32_t grid;           /* workgroups (persistent: waves pull 8x8 tiles from a queue) */
