@@ -119,10 +119,10 @@ typedef struct csg_render_opts {
   int32_t coop_when;    /* 0: hand over only in the frame's tail, once every tile has been taken and a
                            further coop_grace percent of the time that took has passed;
                            1: as soon as the budget is spent */
-  int32_t coop_grace;   /* 0 = default: 950 x waves / tiles percent for one scene, 550 x for a batch (the
-                           tiles in flight when the queue runs dry finish in about a tile's time; handing
-                           out took tiles / waves tile times): 120 for a 1024-lane launch of 1920x1080,
-                           about 3 for the 256-tree batch; < 0 = none; else percent */
+  int32_t coop_grace;   /* 0 = default: 950 x waves / tiles percent (at most 300) for one scene, 550 x for
+                           a batch (the tiles in flight when the queue runs dry finish in about a tile's
+                           time; handing out took tiles / waves tile times): 120 for a 1024-lane launch of
+                           1920x1080, about 3 for the 256-tree batch; < 0 = none; else percent */
 } csg_render_opts;
 
 /* --- scene (replaces CSGTree::Parse + the upload in Raycaster::ChangeSize) --- */
