@@ -54,7 +54,7 @@ struct RenderParams {
   int32_t numTiles;           // queue length (all jobs)
   int32_t stackSlots;         // LDS stack slots per lane (max operator depth of the launch)
   uint32_t totalWaves;        // waves in the grid (sets the default tail grace)
-  uint32_t* queue;            // [0] next tile, [1] waves finished; zero between launches
+  uint32_t* queue;            // work-queue words (layout below); zero between launches
   const uint32_t* tileOrder;  // optional: order in which queue entries map to tiles (plain)
   int32_t morton;             // 1: tiles of a job are visited in Morton order
   // Cooperative evaluation of heavy rays: a ray still running after `budget`

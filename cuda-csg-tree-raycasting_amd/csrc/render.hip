@@ -424,7 +424,7 @@ __device__ __forceinline__ Hit Intersect(const SceneView& sv, const RayF& r, uin
 // launch's start time and the time its last tile was handed out (0 until
 // then), and `lstamps` the workgroup's LDS copy of them (set by its first
 // wave that finds no tile left); the ray goes only when the time since the
-// last tile has reached `grace` percent of the time between the two: the
+// last tile has reached `grace` per mille of the time between the two: the
 // frame's tail.  Lanes of a wave enter together and step together, so
 // `iter` is the same in every active lane and the check (at `budget` steps,
 // then every 32; the global stamps every 256) is wave-uniform.
