@@ -905,9 +905,9 @@ __device__ __forceinline__ void RenderBody(const RenderParams& p) {
   const int lane = tid & 63;
   uint64_t* stamps = reinterpret_cast<uint64_t*>(p.queue + 32);
   uint32_t* tileCtr = p.queue + kTileStripeWord;
-  // this wave's home stripe; once it is empty the wave takes tiles from any
-  // stripe that has some left
-  // the home stripe: XCD-local, or the wave's global index mod the stripes
+  // this wave's home stripe (XCD-local, or the wave's global index mod the
+  // stripes); once it is empty the wave takes tiles from any stripe that has
+  // some left
   uint32_t stripe = __builtin_amdgcn_readfirstlane(
       kXcd ? HomeStripe(blockIdx.x, (blockIdx.x / 8) * (blockDim.x >> 6) + (tid >> 6))
            : (blockIdx.x * (blockDim.x >> 6) + (tid >> 6)) % kQueueStripes);
