@@ -1,8 +1,9 @@
 """Pins for the oracle, independent of the restatement's own formulas.
 
 * Known answers measured on the unmodified reference sources (a host build of
-  RaycastingKernels.cu / CSGTree.cu / Camera.cpp, FP32, recorded in the
-  project survey): hit counts and mean per-ray event counts at 640x360.
+  RaycastingKernels.cu / CSGTree.cu / Camera.cpp, FP32, made once outside
+  this repository, whose figures are written into the tests below): hit
+  counts and mean per-ray event counts at 640x360.
 * Closed forms: cylinder axis = Ry(y) Rx(x) Rz(z) (0,1,0); leaf boxes; camera
   basis; the centre ray's analytic depth on single primitives.
 * An interval classifier that evaluates the CSG expression between the
