@@ -74,9 +74,9 @@ csg_status Upload(csg_scene* s) {
 
 // The render kernels' work queues (tile and heavy-ray counters, time stamps;
 // kQueueBytes) per (device, stream), zero between launches (the last
-// workgroup of each launch zeroes them).  Renders on
-// one stream are ordered, so they can share it; separate streams get separate
-// queues.  Queues live until the process ends.
+// workgroup of each launch zeroes them).  Renders on one stream are ordered,
+// so they can share it; separate streams get separate queues.  Queues live
+// until the process ends.
 std::mutex g_queueMutex;
 std::map<std::pair<int, hipStream_t>, uint32_t*> g_queues;
 
