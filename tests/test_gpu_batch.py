@@ -203,6 +203,41 @@ def test_captured_batch_updates_cameras_and_sizes(gpu):
         _check(new, ref)
 
 
+def test_captured_batch_updates_to_other_scenes_and_job_counts(gpu):
+    """Updating a captured batch to different trees (deeper, with more
+    operators: a larger stack slab and cooperative state, so the graph is
+    captured again), then to fewer jobs, then back: every launch equals the
+    eager renders of the same jobs, and exact mode equals the oracle."""
+    print("\n[case] captured batch updated across scene sets and job counts", flush=True)
+    light, ph = C.light(S.LIGHT_TOWARD), C.phong(*S.PHONG)
+    shallow = _specs_for(S.batch_mix(10, seed=21, resolutions=(32, 64)), light, ph)
+    rng = np.random.default_rng(22)
+    deep_texts = [S.cheese(int(n), seed=int(n)) for n in rng.integers(200, 600, size=6)]
+    deep = [(C.Scene(t), C.camera_look_at(S.orbit_eye(rng, 4.0, 60.0), (0, 0, 0), 60), light, ph, 48, 40,
+             C.alloc_outputs(48, 40)) for t in deep_texts]
+    batch = C.Batch(C.make_jobs(shallow))
+    for specs in (shallow, deep, deep[:2], shallow[:7], deep):
+        for spec in specs:
+            for t in spec[6].values():
+                t.fill_(0)
+        ref = _reference(specs)
+        batch.update(C.make_jobs(specs))
+        batch.launch()
+        torch.cuda.synchronize()
+        _check(specs, ref)
+    batch.close()
+    exact = C.Batch(C.make_jobs(deep[:3]), mode=C.MODE_EXACT)
+    exact.launch()
+    torch.cuda.synchronize()
+    for (sc, cam, l, p, w, h, outs), text in zip(deep[:3], deep_texts[:3]):
+        g = _np(outs)
+        o = O.OracleScene(text).render(C.cam_tuple(cam), S.LIGHT_TOWARD, O.phong8(*S.PHONG), w, h, bits=32)
+        assert np.array_equal(g["prim_id"], o["prim_id"])
+        assert np.array_equal(g["depth"].view(np.uint32), o["depth"].view(np.uint32))
+        assert np.array_equal(g["flags"], o["flags"])
+    exact.close()
+
+
 def test_autotune_keeps_results(gpu):
     light, ph = C.light(S.LIGHT_TOWARD), C.phong(*S.PHONG)
     mix = S.batch_mix(20, seed=17, resolutions=(64, 128))
