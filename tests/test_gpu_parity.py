@@ -164,6 +164,14 @@ def test_launch_plan(gpu):
         assert plan.coop_grace == round(950 * plan.grid * (plan.block // 64) / (240 * 135))
     assert s.plan(block=512).coop_grace == 60
     assert s.plan(coop_grace=30).coop_grace == 30 and s.plan(coop=-1).coop_grace == 0
+    # 1024-lane workgroups only for scenes of 256 operators or more: the 64-sphere cheese (64 operators)
+    # and the Wikipedia scene (7) take 512 lanes, a 255-sphere cheese 512, a 256-sphere one 1024
+    from csgray import scenes as S
+    for text, block in ((S.cheese(64, seed=64), 512), (scene_text("testWikipedia.txt"), 512),
+                        (S.cheese(255, seed=255), 512), (S.cheese(256, seed=256), 1024)):
+        sc = C.Scene(text)
+        assert sc.plan().block == block
+        sc.close()
 
 
 def test_queue_reuse_across_sizes_scenes_and_streams(gpu):
