@@ -147,6 +147,70 @@ def test_fast_vs_fp64_cheese512_spheres_1080p(gpu):
     assert not (g["flags"] & 128).any()
 
 
+def _sampled(d, rows, keys=("prim_id", "flags", "depth", "degen")):
+    """The rendered rows r0:r1:rs of a frame's arrays, flattened."""
+    return {k: d[k][rows[0]:rows[1]:rows[2]].reshape(-1) for k in keys if k in d}
+
+
+def _stacked(parts):
+    return {k: np.concatenate([p[k] for p in parts]) for k in parts[0]}
+
+
+def test_fast_vs_fp64_batch_mix(gpu):
+    """The 256 random trees of the batched benchmark (128^2 to 1024^2, orbit
+    cameras) rendered as one batched launch in fast mode, against the FP64
+    oracle on every 16th row of every job (5.8 M rays)."""
+    from csgray import scenes as S
+    _label("256-tree batch mix", 1)
+    light, ph = C.light(S.LIGHT_TOWARD), C.phong(*S.PHONG)
+    mix = S.batch_mix(256, seed=256)
+    specs = [(C.Scene(text), C.camera_look_at(eye, (0, 0, 0), 60), light, ph, size, size, C.alloc_outputs(size, size))
+             for text, eye, size in mix]
+    C.render_batch(C.make_jobs(specs))
+    torch.cuda.synchronize()
+    gs, os_ = [], []
+    for (text, eye, size), spec in zip(mix, specs):
+        rows = (0, size, 16)
+        g = {k: spec[6][k][rows[0]:rows[1]:rows[2]].cpu().numpy().reshape(-1) for k in ("prim_id", "flags", "depth")}
+        o64 = O.OracleScene(text).render(C.cam_tuple(spec[1]), S.LIGHT_TOWARD, O.phong8(*S.PHONG), size, size,
+                                          bits=64, degen=True, rows=rows)
+        gs.append(g)
+        os_.append(_sampled(o64, rows))
+    g, o64 = _stacked(gs), _stacked(os_)
+    accuracy_report("256-tree batch mix fast (every 16th row)", g, o64)
+    rate, drel, flagged = compare_fp64(g, o64)
+    print(f"agree(unflagged)={rate:.6f} depth_rel_max={drel:.2e} flagged={flagged:.4%}")
+    assert rate >= AGREE_MIN and drel <= DEPTH_REL_TOL
+    for sc, *_ in specs:
+        sc.close()
+
+
+def test_fast_vs_fp64_stress_4k(gpu):
+    """The stress scene (8191 nodes) at 3840x2160, one ray per pixel centre, in
+    fast mode against the FP64 oracle on every 16th row (518 k rays).  The
+    reference's own FP32 arithmetic (the FP32 oracle, which exact mode equals
+    bit for bit) exceeds 1e-4 on a few unflagged end-cap exits of its thin
+    cylinders in this sample (3.05e-4 at most), so the depth bound here is
+    that fast mode is no less accurate than the reference's arithmetic on the
+    same rays, or within 1e-4."""
+    from csgray import scenes as S
+    _label("stress 3840x2160", 1)
+    sname, cam, light, phong, cam6, l3, ph8 = setups()[1]
+    text = S.stress(4095, seed=4095)
+    g = gpu_render(text, cam, light, phong, 3840, 2160, C.MODE_FAST)
+    rows = (0, 2160, 16)
+    oracle = O.OracleScene(text)
+    o64 = oracle.render(cam6, l3, ph8, 3840, 2160, bits=64, degen=True, rows=rows)
+    o32 = oracle.render(cam6, l3, ph8, 3840, 2160, bits=32, rows=rows)
+    accuracy_report("stress 3840x2160 fast (every 16th row)", g, o64, rows=rows)
+    accuracy_report("stress 3840x2160 reference FP32 arithmetic (every 16th row)", o32, o64, rows=rows)
+    rate, drel, flagged = compare_fp64(g, o64, rows=rows)
+    rate32, drel32, _ = compare_fp64(o32, o64, rows=rows)
+    print(f"agree(unflagged)={rate:.6f} depth_rel_max={drel:.2e} (reference FP32 {drel32:.2e}) flagged={flagged:.4%}")
+    assert rate >= AGREE_MIN and rate32 >= AGREE_MIN
+    assert drel <= max(DEPTH_REL_TOL, 1.1 * drel32)
+
+
 def test_launch_plan(gpu):
     s = C.Scene(scene_text("testCheese512.txt"))
     auto = s.plan()
