@@ -189,10 +189,10 @@ def test_fast_vs_fp64_stress_4k(gpu):
     """The stress scene (8191 nodes) at 3840x2160, one ray per pixel centre, in
     fast mode against the FP64 oracle on every 16th row (518 k rays).  The
     reference's own FP32 arithmetic (the FP32 oracle, which exact mode equals
-    bit for bit) exceeds 1e-4 on a few unflagged end-cap exits of its thin
-    cylinders in this sample (3.05e-4 at most), so the depth bound here is
-    that fast mode is no less accurate than the reference's arithmetic on the
-    same rays, or within 1e-4."""
+    bit for bit) exceeds 1e-4 on a few unflagged near-tangent side exits of its
+    thin cylinders, seen almost along their axis, in this sample (3.05e-4 at
+    most), so the depth bound here is that fast mode is no less accurate than
+    the reference's arithmetic on the same rays, or within 1e-4."""
     from csgray import scenes as S
     _label("stress 3840x2160", 1)
     sname, cam, light, phong, cam6, l3, ph8 = setups()[1]
