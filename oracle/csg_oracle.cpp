@@ -69,6 +69,12 @@ namespace oracle {
 //   4 camera up = right x forward (sign flip)           5 strict '<' of RetLIfCloser made '<='
 //   6 SaveLft keeps the current tmin                    7 sphere box half-extent r instead of 2r
 //   8 cylinder cap plane test uses the wrong axis sign
+// and in the hit details and shading:
+//   9 specular not tinted by the object colour   10 Flip negation of the normal dropped
+//  11 Exit negation of the normal dropped         12 N.L used without the clamp at 0
+//  13 V.R used without the clamp at 0             14 reflect(L, N) in place of reflect(-L, N)
+//  15 cylinder side normal P - C (axial part kept)
+//  16 sphere normal P - centre left unnormalised (reflect still normalises its own copy)
 static int g_mutation = 0;
 
 // RaycastingKernels.cu / CSGTree.cuh:38-46
@@ -846,7 +852,8 @@ static PixelOut render_pixel(const Scene<Real>& s, const Camera<Real>& cam, Real
     V3<Real> position = ray.at(hitInfo.t);
     V3<Real> normal;
     if (hitInfo.type == Sphere) {  // :183-200
-      normal = normalize(mk(position.x - p.x, position.y - p.y, position.z - p.z));
+      normal = mk(position.x - p.x, position.y - p.y, position.z - p.z);
+      if (g_mutation != 16) normal = normalize(normal);
     } else if (hitInfo.type == Cylinder) {  // :338-372
       V3<Real> V = mk(p.ax, p.ay, p.az);
       V3<Real> C = mk(p.x, p.y, p.z) - (p.height / 2) * V;
@@ -855,7 +862,7 @@ static PixelOut render_pixel(const Scene<Real>& s, const Camera<Real>& cam, Real
       else if (hitInfo.hit & Flag2) normal = V;
       else {
         Real m = dot(ray.direction, V) * hitInfo.t + dot(OC, V);
-        normal = normalize(position - C - m * V);
+        normal = normalize(g_mutation == 15 ? position - C : position - C - m * V);
       }
     } else {  // cube, :436-457
       V3<Real> C = mk(p.x, p.y, p.z);
@@ -865,22 +872,26 @@ static PixelOut render_pixel(const Scene<Real>& s, const Camera<Real>& cam, Real
       normal = normalize(mk(trunc_int(PC.x / halfSize * bias), trunc_int(PC.y / halfSize * bias),
                             trunc_int(PC.z / halfSize * bias)));
     }
-    if (hitInfo.hit & Flip) normal = -normal;
-    if (hitInfo.hit & Exit) normal = -normal;
+    if ((hitInfo.hit & Flip) && g_mutation != 10) normal = -normal;
+    if ((hitInfo.hit & Exit) && g_mutation != 11) normal = -normal;
     // Phong, :62-105
     V3<Real> color = mk(p.r, p.g, p.b);
     V3<Real> lightColor = mk<Real>(1, 1, 1);
     Real ka = phong[0], kd = phong[1], ks = phong[2], shininess = phong[3];
     V3<Real> lightDir = mk(lightN[0], lightN[1], lightN[2]);
     V3<Real> viewDir = normalize(mk(cam.x, cam.y, cam.z) - position);
-    V3<Real> reflectDir = reflect(-lightDir, normal);
+    V3<Real> reflectDir = reflect(g_mutation == 14 ? lightDir : -lightDir, normal);
     V3<Real> ambient = ka * lightColor;
-    Real diff = std::fmax(dot(normal, lightDir), Real(0));
+    Real diff = g_mutation == 12 ? dot(normal, lightDir) : std::fmax(dot(normal, lightDir), Real(0));
     V3<Real> diffuse = kd * diff * lightColor;
-    Real spec = std::pow(std::fmax(dot(viewDir, reflectDir), Real(0)), shininess);
+    Real vr = dot(viewDir, reflectDir);
+    Real spec = std::pow(g_mutation == 13 ? vr : std::fmax(vr, Real(0)), shininess);
     V3<Real> specular = ks * spec * lightColor;
     V3<Real> fc = mk(color.x * (ambient.x + diffuse.x + specular.x), color.y * (ambient.y + diffuse.y + specular.y),
                      color.z * (ambient.z + diffuse.z + specular.z));
+    if (g_mutation == 9)
+      fc = mk(color.x * (ambient.x + diffuse.x) + specular.x, color.y * (ambient.y + diffuse.y) + specular.y,
+              color.z * (ambient.z + diffuse.z) + specular.z);
     o.rgba[0] = (float)std::fmin(std::fmax(fc.x, Real(0)), Real(1));
     o.rgba[1] = (float)std::fmin(std::fmax(fc.y, Real(0)), Real(1));
     o.rgba[2] = (float)std::fmin(std::fmax(fc.z, Real(0)), Real(1));
