@@ -95,7 +95,7 @@ class RequestStats(ctypes.Structure):
 class ServerStats(ctypes.Structure):
     _fields_ = [("submitted", ctypes.c_int64), ("completed", ctypes.c_int64), ("rejected", ctypes.c_int64),
                 ("batches", ctypes.c_int64), ("rays_done", ctypes.c_int64), ("bytes_in_use", ctypes.c_int64),
-                ("peak_bytes_in_use", ctypes.c_int64)]
+                ("peak_bytes_in_use", ctypes.c_int64), ("workspace_bytes", ctypes.c_int64)]
 
 
 class CsgError(RuntimeError):

@@ -75,8 +75,9 @@ csg_status Upload(csg_scene* s) {
 // The render kernels' work queues (tile and heavy-ray counters, time stamps;
 // kQueueBytes) per (device, stream), zero between launches (the last
 // workgroup of each launch zeroes them).  Renders on one stream are ordered,
-// so they can share it; separate streams get separate queues.  Queues live
-// until the process ends.
+// so they can share it; separate streams get separate queues.  A stream's
+// queue, heavy-ray list and job table live until ReleaseStreamWorkspace (the
+// server's streams) or the process ends.
 std::mutex g_queueMutex;
 std::map<std::pair<int, hipStream_t>, uint32_t*> g_queues;
 
@@ -503,6 +504,45 @@ csg_status UploadJobs(int dev, hipStream_t stream, const csgray::Job* jobs, size
   if (e != hipSuccess) return FailHip(e, "hipEventRecord");
   *out = static_cast<const csgray::Job*>(t.dev);
   return CSG_OK;
+}
+
+size_t StreamWorkspaceBytes(int32_t maxJobs) {
+  const size_t table = std::max<size_t>(sizeof(csgray::Job) * static_cast<size_t>(std::max(maxJobs, 1)), 64 * 1024);
+  return Align256(csgray::kQueueBytes) + Align256(csgray::kHeavyBytes) + Align256(table);
+}
+
+csg_status ReleaseStreamWorkspace(int dev, hipStream_t stream) {
+  csg_status st = CSG_OK;
+  const std::pair<int, hipStream_t> key{dev, stream};
+  {
+    std::lock_guard<std::mutex> lock(g_queueMutex);
+    auto it = g_queues.find(key);
+    if (it != g_queues.end()) {
+      if (hipFree(it->second) != hipSuccess) st = Fail(CSG_E_HIP, "hipFree of a queue failed");
+      g_queues.erase(it);
+    }
+  }
+  {
+    std::lock_guard<std::mutex> lock(g_heavyMutex);
+    auto it = g_heavy.find(key);
+    if (it != g_heavy.end()) {
+      if (hipFree(it->second) != hipSuccess) st = Fail(CSG_E_HIP, "hipFree of a heavy-ray list failed");
+      g_heavy.erase(it);
+    }
+  }
+  {
+    std::lock_guard<std::mutex> lock(g_jobsMutex);
+    auto it = g_jobTables.find(key);
+    if (it != g_jobTables.end()) {
+      JobTable& t = it->second;
+      if (t.copied) (void)hipEventSynchronize(t.copied);
+      if (t.dev && hipFree(t.dev) != hipSuccess) st = Fail(CSG_E_HIP, "hipFree of a job table failed");
+      if (t.host) (void)hipHostFree(t.host);
+      if (t.copied) (void)hipEventDestroy(t.copied);
+      g_jobTables.erase(it);
+    }
+  }
+  return st;
 }
 }  // namespace rt
 }  // namespace csgray

@@ -189,12 +189,12 @@ __device__ __forceinline__ void Slab(const RayF& r, float lx, float ly, float lz
 }
 
 // isBVHNodeHit (:718-757).  The caller turns a miss into hit = Miss.
-// Fast mode computes each slab distance as one fma, l * (1/d) + (-o/d).
-// Where a direction component is exactly zero that gives NaN in place of
-// the reference's +-inf, and fminf/fmaxf then ignore the axis: the test can
-// only pass more rays than the reference's, which costs a visit and never
-// changes a result (a box only culls; the cube primitive, whose slab is its
-// answer, keeps the reference's form).
+// Fast mode computes each slab distance as one fma, l * (1/d) + (-o/d), with
+// the per-ray factors from MakeRay.  Where a direction component is exactly
+// zero, 1/d is taken as +-1e30 (SlabRcp): the slab limits become +-huge of
+// the sign of l - o, as the reference's (l - o) / 0 = +-inf decides them, so
+// a box beside a ray parallel to its face is culled as in the reference (with
+// a true infinity, -o/d would be NaN at o = 0 and the axis ignored).
 template <bool kFast>
 __device__ __forceinline__ bool BoxHit(const RayF& r, float lx, float ly, float lz, float hx, float hy, float hz,
                                        float tmin) {
@@ -214,15 +214,27 @@ __device__ __forceinline__ bool BoxHit(const RayF& r, float lx, float ly, float 
   return !(tmx < 0) & !(tmn > tmx) & !((tmn <= tmin) & (tmx <= tmin));
 }
 
-// sphereHit (:135-181).  Fast mode takes the hardware square root (about
-// 1 ulp) in place of the correctly rounded one.
+// sphereHit (:135-181).  Fast mode forms the discriminant from the ray's
+// closest approach to the centre, r^2 - |oc - b d|^2 (Haines, Guenter,
+// Akenine-Moller, "Precision Improvements for Ray/Sphere Intersection", Ray
+// Tracing Gems, 2019, ch. 7): the same quantity as the reference's
+// b^2 - (|oc|^2 - r^2) for a unit direction, without that form's
+// cancellation of two numbers of the size |oc|^2, so its error scales with
+// the sphere and not with its distance.  It also takes the hardware square
+// root (about 1 ulp).
 template <bool kFast>
 __device__ __forceinline__ Hit SphereHit(const RayF& r, const float4& a, uint32_t prim, float tmin, Hit h) {
   h.bits = HitBits(kMissBit, 0, prim);
   const float ocx = r.ox - a.x, ocy = r.oy - a.y, ocz = r.oz - a.z;
   const float b = ocx * r.dx + ocy * r.dy + ocz * r.dz;
-  const float c = (ocx * ocx + ocy * ocy + ocz * ocz) - a.w * a.w;
-  const float disc = b * b - c;
+  float disc;
+  if (kFast) {
+    const float fx = fmaf(-b, r.dx, ocx), fy = fmaf(-b, r.dy, ocy), fz = fmaf(-b, r.dz, ocz);
+    disc = a.w * a.w - (fx * fx + fy * fy + fz * fz);
+  } else {
+    const float c = (ocx * ocx + ocy * ocy + ocz * ocz) - a.w * a.w;
+    disc = b * b - c;
+  }
   if (disc < 0) return h;
   const float sq = kFast ? __builtin_amdgcn_sqrtf(disc) : sqrtf(disc);
   float temp = (-b - sq);
@@ -240,15 +252,20 @@ __device__ __forceinline__ Hit SphereHit(const RayF& r, const float4& a, uint32_
 // cylinderHit (:202-336).  Both roots go through the same cap logic, as in
 // the reference's duplicated blocks (:238-266, :273-301).
 //
-// kCylinderF64 (fast mode only): when the discriminant is within
-// kNearTangent of b^2 (a grazing ray, where FP32 cancellation in b^2 - A c
-// decides hit or miss and moves the roots), solve the quadratic again in FP64
-// from the same FP32 ray.  Measured off: it left hit-id agreement with the
-// FP64 oracle unchanged on the Wikipedia view (its mismatches are coplanar
-// ties) and its registers cost 10% on the 256-tree batch.
-constexpr float kNearTangent = 1e-3f;
-constexpr bool kCylinderF64 = false;
-
+// Fast mode solves the side's quadratic in the plane across the axis, from
+// the ray's closest approach to the axis (the sphere's closest-approach
+// discriminant carried to that plane).  With e = d x V and g = OC x V (d and
+// OC turned a quarter about V: lengths and dot products as their components
+// across the axis), A = |e|^2, b = e.g, the closest approach lies at
+// tc = -b / A, at distance |g + tc e| from the axis, and the roots are
+// tc -+ sqrt((r^2 - |g + tc e|^2) / A): the reference's (-b -+ sqrt(b^2 - A c)) / A
+// with c = |g|^2 - r^2.  The reference forms c as |OC|^2 - (OC.V)^2 - r^2,
+// which cancels two numbers of the size |OC|^2 and amplifies the rounding of
+// the FP32 axis's length (|OC|^2 (1 - |V|^2)); for thin cylinders seen nearly
+// along their axis that moved depths by 3e-4 relative, while the form here
+// keeps them near 1e-6.  Where the reference's clamp A = 1e-5 applies (rays
+// within 0.18 degrees of the axis) the clamped quadratic is kept:
+// (b^2 - A c) / A = -b tc - c.
 template <bool kFast>
 __device__ __forceinline__ Hit CylinderHit(const RayF& r, const float4& a, const float4& bx, uint32_t prim,
                                            float tmin, Hit h) {
@@ -258,41 +275,36 @@ __device__ __forceinline__ Hit CylinderHit(const RayF& r, const float4& a, const
   const float Cx = a.x - hh * vx, Cy = a.y - hh * vy, Cz = a.z - hh * vz;
   const float OCx = r.ox - Cx, OCy = r.oy - Cy, OCz = r.oz - Cz;
   const float dV = r.dx * vx + r.dy * vy + r.dz * vz;  // dot(d, V)
-  const float A = fmaxf(1 - dV * dV, 0.00001f);
   const float ocV = OCx * vx + OCy * vy + OCz * vz;  // dot(OC, V)
-  const float c = (OCx * OCx + OCy * OCy + OCz * OCz) - ocV * ocV - radius * radius;
-  const float dOC = r.dx * OCx + r.dy * OCy + r.dz * OCz;
-  const float b = (dOC - dV * ocV);
-  const float disc = b * b - A * c;
-  float t1, t2, m1, m2;
-  if (kFast && kCylinderF64 && fabsf(disc) <= kNearTangent * (b * b)) {
-    const double Cxd = static_cast<double>(a.x) - static_cast<double>(hh) * vx;
-    const double Cyd = static_cast<double>(a.y) - static_cast<double>(hh) * vy;
-    const double Czd = static_cast<double>(a.z) - static_cast<double>(hh) * vz;
-    const double ox = static_cast<double>(r.ox) - Cxd, oy = static_cast<double>(r.oy) - Cyd;
-    const double oz = static_cast<double>(r.oz) - Czd;
-    const double dVd = static_cast<double>(r.dx) * vx + static_cast<double>(r.dy) * vy + static_cast<double>(r.dz) * vz;
-    const double Ad = fmax(1.0 - dVd * dVd, 0.00001);
-    const double ocVd = ox * vx + oy * vy + oz * vz;
-    const double cd = (ox * ox + oy * oy + oz * oz) - ocVd * ocVd - static_cast<double>(radius) * radius;
-    const double bd = (static_cast<double>(r.dx) * ox + static_cast<double>(r.dy) * oy + static_cast<double>(r.dz) * oz) -
-                      dVd * ocVd;
-    const double discd = bd * bd - Ad * cd;
-    if (discd < 0) return h;
-    const double sqd = sqrt(discd);
-    const double t1d = (-bd - sqd) / Ad, t2d = (-bd + sqd) / Ad;
-    t1 = static_cast<float>(t1d);
-    t2 = static_cast<float>(t2d);
-    m1 = static_cast<float>(t1d * dVd + ocVd);
-    m2 = static_cast<float>(t2d * dVd + ocVd);
+  float t1, t2;
+  if (kFast) {
+    const float ex = r.dy * vz - r.dz * vy, ey = r.dz * vx - r.dx * vz, ez = r.dx * vy - r.dy * vx;
+    const float gx = OCy * vz - OCz * vy, gy = OCz * vx - OCx * vz, gz = OCx * vy - OCy * vx;
+    const float A0 = ex * ex + ey * ey + ez * ez;
+    const float A = fmaxf(A0, 0.00001f);
+    const float b = ex * gx + ey * gy + ez * gz;
+    const float tc = -b / A;
+    const float fx = fmaf(tc, ex, gx), fy = fmaf(tc, ey, gy), fz = fmaf(tc, ez, gz);
+    const float rr = radius * radius;
+    const float dp = A0 >= 0.00001f ? rr - (fx * fx + fy * fy + fz * fz)
+                                    : fmaf(-b, tc, rr - (gx * gx + gy * gy + gz * gz));
+    if (dp < 0) return h;
+    const float s = __builtin_amdgcn_sqrtf(dp * __builtin_amdgcn_rcpf(A));
+    t1 = tc - s;
+    t2 = tc + s;
   } else {
+    const float A = fmaxf(1 - dV * dV, 0.00001f);
+    const float c = (OCx * OCx + OCy * OCy + OCz * OCz) - ocV * ocV - radius * radius;
+    const float dOC = r.dx * OCx + r.dy * OCy + r.dz * OCz;
+    const float b = (dOC - dV * ocV);
+    const float disc = b * b - A * c;
     if (disc < 0) return h;
     const float sq = sqrtf(disc);
     t1 = (-b - sq) / (A);
     t2 = (-b + sq) / (A);
-    m1 = t1 * dV + ocV;
-    m2 = t2 * dV + ocV;
   }
+  const float m1 = t1 * dV + ocV;
+  const float m2 = t2 * dV + ocV;
   if ((m1 < 0 && m2 < 0) || (m1 > height && m2 > height)) return h;
 
   // dot(d, -V), dot(-OC, -V) and dot(-OCmMax, V) written out as the
@@ -623,6 +635,13 @@ __device__ __forceinline__ void Normalize3(float& x, float& y, float& z) {
   z = inv * z;
 }
 
+// Reciprocal of a direction component for fast-mode slabs: the hardware
+// reciprocal, or +-1e30 with the component's sign where it is zero (or below
+// 1e-30), so that -o/d stays finite and l/d - o/d keeps the sign of l - o.
+__device__ __forceinline__ float SlabRcp(float d) {
+  return fabsf(d) < 1e-30f ? copysignf(1e30f, d) : __builtin_amdgcn_rcpf(d);
+}
+
 // Ray for pixel (x, y) (RaycastingKernels.cu:11-27, Ray.cuh:12-18).  The
 // reference samples the pixel at offset 0.5; with jitter the offsets come
 // from JitterHash.  The direction is normalised with IEEE square root and
@@ -652,9 +671,9 @@ __device__ __forceinline__ RayF MakeRay(const Job& jb, int x, int y, float ox, f
   r.dx = dx * invLen;
   r.dy = dy * invLen;
   r.dz = dz * invLen;
-  r.ix = kFast ? __builtin_amdgcn_rcpf(r.dx) : 1.0f / r.dx;
-  r.iy = kFast ? __builtin_amdgcn_rcpf(r.dy) : 1.0f / r.dy;
-  r.iz = kFast ? __builtin_amdgcn_rcpf(r.dz) : 1.0f / r.dz;
+  r.ix = kFast ? SlabRcp(r.dx) : 1.0f / r.dx;
+  r.iy = kFast ? SlabRcp(r.dy) : 1.0f / r.dy;
+  r.iz = kFast ? SlabRcp(r.dz) : 1.0f / r.dz;
   r.nox = -r.ox * r.ix;
   r.noy = -r.oy * r.iy;
   r.noz = -r.oz * r.iz;

@@ -51,5 +51,13 @@ void PrepareCoop(const LaunchConfig& cfg, int32_t maxOps, int32_t maxLeaves, uin
 // Copies a job table to the (device, stream) table buffer in stream order.
 csg_status UploadJobs(int dev, hipStream_t stream, const Job* jobs, size_t n, const Job** out);
 
+// Device bytes of one stream's workspace (queue, heavy-ray list with its
+// resume area, job table for up to `maxJobs` jobs).
+size_t StreamWorkspaceBytes(int32_t maxJobs);
+
+// Frees the queue, heavy-ray list and job table of (device, stream); the
+// caller has synchronised the stream and renders no more on it.
+csg_status ReleaseStreamWorkspace(int dev, hipStream_t stream);
+
 }  // namespace rt
 }  // namespace csgray

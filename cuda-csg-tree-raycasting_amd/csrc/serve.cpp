@@ -1,12 +1,16 @@
 // Serving loop: resident scenes, a FIFO request queue, admission by a device
 // memory budget, and continuous batching over several streams.
 //
-// A worker thread owns every HIP call after creation.  Each pass it retires
+// A worker thread owns every launch after creation.  Each pass it retires
 // batches whose completion event has fired (their requests become done), then
 // for every idle stream takes queued requests in arrival order while their
 // outputs fit the budget, and launches them as one batched render.  A request
 // that arrives while batches run therefore starts as soon as a stream frees
 // up, overlapping the batches still on the device.
+//
+// The budget covers everything the server holds on the device: each stream's
+// workspace (tile queue, heavy-ray list with its resume area, job table),
+// allocated when the server is created, and the requests' output blocks.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -39,10 +43,17 @@ size_t OutputBytes(int64_t pixels) {
 
 enum State { kQueued = 0, kRunning = 1, kDone = 2 };
 
+// An output block, and the event recorded after the last csg_server_copy out
+// of it (null if none): the block is not written again before that copy.
+struct Block {
+  void* mem = nullptr;
+  hipEvent_t copied = nullptr;
+};
+
 struct Request {
   csg_request req{};
   size_t bytes = 0;
-  void* mem = nullptr;
+  Block block;
   csg_outputs out{};
   State state = kQueued;
   csg_request_stats stats{};
@@ -51,7 +62,8 @@ struct Request {
 struct Slot {
   hipStream_t stream = nullptr;
   hipEvent_t done = nullptr;
-  std::vector<int64_t> tickets;  // empty = idle
+  std::vector<int64_t> tickets;      // empty = idle
+  std::vector<hipEvent_t> waited;    // copy events this slot's batch waits on (recycled when it retires)
 };
 
 }  // namespace
@@ -67,8 +79,11 @@ struct csg_server {
   std::deque<int64_t> queue;
   std::unordered_map<int64_t, Request> reqs;
   int64_t nextTicket = 1;
-  int64_t used = 0, cached = 0;
-  std::multimap<size_t, void*> cache;  // released output blocks by size
+  int64_t used = 0, cached = 0;   // output bytes held by requests / in the cache
+  int64_t workspace = 0;          // the streams' workspaces
+  int64_t outputBudget = 0;       // budget_bytes - workspace
+  std::multimap<size_t, Block> cache;  // released output blocks by size
+  std::vector<hipEvent_t> eventPool;
   std::vector<Slot> slots;
   csg_server_stats stats{};
   bool stop = false;
@@ -80,43 +95,59 @@ namespace {
 
 double Since(const csg_server* s) { return std::chrono::duration<double>(Clock::now() - s->t0).count(); }
 
+void ReturnEvent(csg_server* s, hipEvent_t e) {
+  if (e) s->eventPool.push_back(e);
+}
+
 // Takes an output block of `bytes` from the cache or the device; the caller
 // holds the lock and has checked the budget.
-void* TakeBlock(csg_server* s, size_t bytes) {
+Block TakeBlock(csg_server* s, size_t bytes) {
   auto it = s->cache.find(bytes);
   if (it != s->cache.end()) {
-    void* p = it->second;
+    const Block b = it->second;
     s->cache.erase(it);
     s->cached -= static_cast<int64_t>(bytes);
-    return p;
+    return b;
   }
   // make room: cached blocks of other sizes count against the budget too
-  while (!s->cache.empty() && s->used + s->cached + static_cast<int64_t>(bytes) > s->cfg.budget_bytes) {
+  while (!s->cache.empty() && s->used + s->cached + static_cast<int64_t>(bytes) > s->outputBudget) {
     auto victim = s->cache.begin();
     s->cached -= static_cast<int64_t>(victim->first);
-    (void)hipFree(victim->second);
+    if (victim->second.copied) (void)hipEventSynchronize(victim->second.copied);
+    ReturnEvent(s, victim->second.copied);
+    (void)hipFree(victim->second.mem);
     s->cache.erase(victim);
   }
-  void* p = nullptr;
-  if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
-  return p;
+  Block b;
+  if (hipMalloc(&b.mem, bytes) != hipSuccess) b.mem = nullptr;
+  return b;
+}
+
+// Marks a request done with `status`; the lock is held.
+void Finish(csg_server* s, Request& r, csg_status status, double t) {
+  r.state = kDone;
+  r.stats.done_s = t;
+  if (r.stats.status == CSG_OK) r.stats.status = status;
+  s->stats.completed++;
+  if (r.stats.status == CSG_OK) s->stats.rays_done += static_cast<int64_t>(r.req.width) * r.req.height;
 }
 
 void Retire(csg_server* s) {
   bool any = false;
   for (Slot& sl : s->slots) {
     if (sl.tickets.empty()) continue;
-    if (hipEventQuery(sl.done) != hipSuccess) continue;
+    const hipError_t q = hipEventQuery(sl.done);
+    if (q == hipErrorNotReady) continue;
+    // anything but "not ready" ends the batch; an error fails its requests
+    const csg_status st = q == hipSuccess ? CSG_OK : CSG_E_HIP;
     const double t = Since(s);
     for (int64_t tk : sl.tickets) {
       auto it = s->reqs.find(tk);
-      if (it == s->reqs.end()) continue;
-      it->second.state = kDone;
-      it->second.stats.done_s = t;
-      s->stats.completed++;
-      s->stats.rays_done += static_cast<int64_t>(it->second.req.width) * it->second.req.height;
+      if (it != s->reqs.end()) Finish(s, it->second, st, t);
     }
     sl.tickets.clear();
+    for (hipEvent_t e : sl.waited) ReturnEvent(s, e);
+    sl.waited.clear();
     any = true;
   }
   if (any) s->doneCv.notify_all();
@@ -127,14 +158,14 @@ void Dispatch(csg_server* s, Slot& sl) {
   std::vector<int64_t> take;
   while (!s->queue.empty() && static_cast<int32_t>(take.size()) < s->cfg.max_batch_jobs) {
     Request& r = s->reqs[s->queue.front()];
-    if (s->used + static_cast<int64_t>(r.bytes) > s->cfg.budget_bytes) break;  // wait for releases
-    void* mem = TakeBlock(s, r.bytes);
-    if (!mem) break;
+    if (s->used + static_cast<int64_t>(r.bytes) > s->outputBudget) break;  // wait for releases
+    const Block blk = TakeBlock(s, r.bytes);
+    if (!blk.mem) break;
     s->used += static_cast<int64_t>(r.bytes);
-    s->stats.peak_bytes_in_use = std::max(s->stats.peak_bytes_in_use, s->used);
-    r.mem = mem;
+    s->stats.peak_bytes_in_use = std::max(s->stats.peak_bytes_in_use, s->used + s->workspace);
+    r.block = blk;
     const int64_t px = static_cast<int64_t>(r.req.width) * r.req.height;
-    char* b = static_cast<char*>(mem);
+    char* b = static_cast<char*>(blk.mem);
     r.out.rgba = reinterpret_cast<float*>(b);
     r.out.depth = reinterpret_cast<float*>(b + Align256(16 * px));
     r.out.prim_id = reinterpret_cast<int32_t*>(b + Align256(16 * px) + Align256(4 * px));
@@ -143,23 +174,50 @@ void Dispatch(csg_server* s, Slot& sl) {
     s->queue.pop_front();
   }
   if (take.empty()) return;
-  std::vector<csgray::Job> jobs(take.size());
+  const double t = Since(s);
+  std::vector<csgray::Job> jobs;
+  std::vector<int64_t> run;
+  jobs.reserve(take.size());
   int64_t base = 0;
-  int32_t depth = 1;
+  int32_t depth = 1, maxOps = 1, maxLeaves = 1;
   csg_render_opts opts{};
   opts.mode = s->cfg.mode;
-  const double t = Since(s);
-  for (size_t i = 0; i < take.size(); i++) {
-    Request& r = s->reqs[take[i]];
+  bool failed = false;
+  for (int64_t tk : take) {
+    Request& r = s->reqs[tk];
     const csg_scene* sc = s->scenes[r.req.scene_id];
-    (void)BuildJob(sc, &r.req.camera, &r.req.light, &r.req.phong, r.req.width, r.req.height, &r.out, &opts,
-                   s->device, base, &jobs[i]);
-    base += jobs[i].tiles;
-    depth = std::max(depth, sc->dev.opDepth);
-    r.state = kRunning;
+    csgray::Job jb;
+    const csg_status js = BuildJob(sc, &r.req.camera, &r.req.light, &r.req.phong, r.req.width, r.req.height, &r.out,
+                                   &opts, s->device, base, &jb);
     r.stats.start_s = t;
-    r.stats.batch_jobs = static_cast<int32_t>(take.size());
+    if (js != CSG_OK) {
+      // a request that fails validation is not launched; it is done at once
+      r.stats.status = js;
+      Finish(s, r, js, t);
+      failed = true;
+      continue;
+    }
+    // the block's last copy out (a released request's) completes before this batch writes it
+    if (r.block.copied) {
+      if (hipStreamWaitEvent(sl.stream, r.block.copied, 0) == hipSuccess) {
+        sl.waited.push_back(r.block.copied);
+      } else {
+        (void)hipEventSynchronize(r.block.copied);
+        ReturnEvent(s, r.block.copied);
+      }
+      r.block.copied = nullptr;
+    }
+    jobs.push_back(jb);
+    run.push_back(tk);
+    base += jb.tiles;
+    depth = std::max(depth, sc->dev.opDepth);
+    maxOps = std::max(maxOps, sc->dev.nOps);
+    maxLeaves = std::max(maxLeaves, sc->dev.opLeaves);
+    r.state = kRunning;
   }
+  if (failed) s->doneCv.notify_all();
+  if (run.empty()) return;
+  for (int64_t tk : run) s->reqs[tk].stats.batch_jobs = static_cast<int32_t>(run.size());
   csgray::RenderParams p{};
   csg_status st = UploadJobs(s->device, sl.stream, jobs.data(), jobs.size(), &p.jobs);
   csgray::LaunchShape shape;
@@ -177,21 +235,15 @@ void Dispatch(csg_server* s, Slot& sl) {
     p.queue = queue;
     csgray::FitGridToTiles(&shape, base);
     p.totalWaves = static_cast<uint32_t>(shape.grid) * static_cast<uint32_t>(shape.block / 64);
-    int32_t maxOps = 1, maxLeaves = 1;
-    for (int64_t tk : take) {
-      const csgray::DevScene& d = s->scenes[s->reqs[tk].req.scene_id]->dev;
-      maxOps = std::max(maxOps, d.nOps);
-      maxLeaves = std::max(maxLeaves, d.opLeaves);
-    }
     PrepareCoop(s->launch, maxOps, maxLeaves, heavy, shape, &p);
     if (csgray::LaunchRender(p, s->launch, shape, sl.stream) != hipSuccess) st = CSG_E_HIP;
   }
   if (st == CSG_OK && hipEventRecord(sl.done, sl.stream) != hipSuccess) st = CSG_E_HIP;
-  // A failed launch still retires its requests (outputs undefined) so that
-  // no client blocks forever; the error is visible as a short batch time.
+  // A failed launch still retires its requests (outputs undefined, status
+  // set) so that no client blocks forever.
   if (st != CSG_OK) (void)hipEventRecord(sl.done, sl.stream);
-  for (int64_t tk : take) s->reqs[tk].stats.status = st;
-  sl.tickets = std::move(take);
+  for (int64_t tk : run) s->reqs[tk].stats.status = st;
+  sl.tickets = std::move(run);
   s->stats.batches++;
 }
 
@@ -218,6 +270,18 @@ void WorkerMain(csg_server* s) {
   }
 }
 
+void DestroyStreams(csg_server* s) {
+  for (Slot& sl : s->slots) {
+    if (sl.stream) {
+      (void)hipStreamSynchronize(sl.stream);
+      (void)ReleaseStreamWorkspace(s->device, sl.stream);
+      (void)hipStreamDestroy(sl.stream);
+    }
+    if (sl.done) (void)hipEventDestroy(sl.done);
+    for (hipEvent_t e : sl.waited) (void)hipEventDestroy(e);
+  }
+}
+
 }  // namespace
 
 extern "C" {
@@ -240,21 +304,31 @@ csg_status csg_server_create(const csg_server_config* cfg, csg_server** out) {
   o.coop_grace = s->cfg.coop_grace;
   csg_status st = ParseOpts(&o, &s->launch);
   if (st == CSG_OK && hipGetDevice(&s->device) != hipSuccess) st = Fail(CSG_E_HIP, "hipGetDevice failed");
+  s->workspace = static_cast<int64_t>(StreamWorkspaceBytes(s->cfg.max_batch_jobs)) * s->cfg.streams;
+  s->outputBudget = s->cfg.budget_bytes - s->workspace;
+  if (st == CSG_OK && s->outputBudget <= 0)
+    st = Fail(CSG_E_BUDGET, "csg_server_create: the budget does not cover the streams' workspaces (" +
+                                std::to_string(s->workspace) + " bytes)");
   s->slots.resize(static_cast<size_t>(s->cfg.streams));
   for (Slot& sl : s->slots) {
     if (st != CSG_OK) break;
     if (hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreateWithFlags(&sl.done, hipEventDisableTiming) != hipSuccess)
+        hipEventCreateWithFlags(&sl.done, hipEventDisableTiming) != hipSuccess) {
       st = Fail(CSG_E_HIP, "stream or event creation failed");
+      break;
+    }
+    // the stream's workspace now, inside the budget (not at its first batch)
+    uint32_t* q = nullptr;
+    uint2* h = nullptr;
+    st = QueueFor(s->device, sl.stream, &q);
+    if (st == CSG_OK) st = HeavyFor(s->device, sl.stream, &h);
   }
   if (st != CSG_OK) {
-    for (Slot& sl : s->slots) {
-      if (sl.done) (void)hipEventDestroy(sl.done);
-      if (sl.stream) (void)hipStreamDestroy(sl.stream);
-    }
+    DestroyStreams(s);
     delete s;
     return st;
   }
+  s->stats.peak_bytes_in_use = s->workspace;
   s->t0 = Clock::now();
   s->worker = std::thread(WorkerMain, s);
   *out = s;
@@ -280,9 +354,10 @@ csg_status csg_server_submit(csg_server* s, const csg_request* rq, int64_t* tick
   std::lock_guard<std::mutex> lock(s->mu);
   if (rq->scene_id < 0 || rq->scene_id >= static_cast<int32_t>(s->scenes.size()))
     return Fail(CSG_E_ARG, "csg_server_submit: unknown scene id");
-  if (static_cast<int64_t>(bytes) > s->cfg.budget_bytes) {
+  if (static_cast<int64_t>(bytes) > s->outputBudget) {
     s->stats.rejected++;
-    return Fail(CSG_E_BUDGET, "csg_server_submit: the request's outputs exceed the memory budget");
+    return Fail(CSG_E_BUDGET, "csg_server_submit: the request's outputs exceed the memory budget left beside the "
+                              "streams' workspaces");
   }
   const int64_t t = s->nextTicket++;
   Request& r = s->reqs[t];
@@ -318,18 +393,17 @@ csg_status csg_server_wait(csg_server* s, int64_t ticket, csg_outputs* out, csg_
 
 csg_status csg_server_copy(csg_server* s, int64_t ticket, const csg_outputs* dst, void* stream) {
   if (!s || !dst) return Fail(CSG_E_ARG, "csg_server_copy: NULL argument");
-  csg_outputs src{};
-  int64_t px = 0;
-  {
-    std::lock_guard<std::mutex> lock(s->mu);
-    auto it = s->reqs.find(ticket);
-    if (it == s->reqs.end()) return Fail(CSG_E_ARG, "csg_server_copy: unknown ticket");
-    if (it->second.state != kDone) return Fail(CSG_E_ARG, "csg_server_copy: request not done");
-    src = it->second.out;
-    px = static_cast<int64_t>(it->second.req.width) * it->second.req.height;
-  }
+  // Held through the copies: the event recorded after them goes with the
+  // block, and a batch that reuses the block waits for it.
+  std::lock_guard<std::mutex> lock(s->mu);
+  auto it = s->reqs.find(ticket);
+  if (it == s->reqs.end()) return Fail(CSG_E_ARG, "csg_server_copy: unknown ticket");
+  Request& r = it->second;
+  if (r.state != kDone) return Fail(CSG_E_ARG, "csg_server_copy: request not done");
+  if (!r.block.mem) return Fail(CSG_E_ARG, "csg_server_copy: the request has no outputs (it failed)");
+  const int64_t px = static_cast<int64_t>(r.req.width) * r.req.height;
   const hipStream_t hs = static_cast<hipStream_t>(stream);
-  const void* from[4] = {src.rgba, src.depth, src.prim_id, src.flags};
+  const void* from[4] = {r.out.rgba, r.out.depth, r.out.prim_id, r.out.flags};
   void* to[4] = {dst->rgba, dst->depth, dst->prim_id, dst->flags};
   const int64_t sz[4] = {16 * px, 4 * px, 4 * px, px};
   for (int k = 0; k < 4; k++) {
@@ -337,6 +411,18 @@ csg_status csg_server_copy(csg_server* s, int64_t ticket, const csg_outputs* dst
     const hipError_t e = hipMemcpyAsync(to[k], from[k], static_cast<size_t>(sz[k]), hipMemcpyDeviceToDevice, hs);
     if (e != hipSuccess) return FailHip(e, "hipMemcpyAsync of request outputs");
   }
+  if (!r.block.copied) {
+    if (!s->eventPool.empty()) {
+      r.block.copied = s->eventPool.back();
+      s->eventPool.pop_back();
+    } else if (hipEventCreateWithFlags(&r.block.copied, hipEventDisableTiming) != hipSuccess) {
+      r.block.copied = nullptr;
+      const hipError_t e = hipStreamSynchronize(hs);  // no event: finish the copies here
+      return e == hipSuccess ? CSG_OK : FailHip(e, "hipStreamSynchronize");
+    }
+  }
+  const hipError_t e = hipEventRecord(r.block.copied, hs);
+  if (e != hipSuccess) return FailHip(e, "hipEventRecord after the copies");
   return CSG_OK;
 }
 
@@ -346,8 +432,8 @@ csg_status csg_server_release(csg_server* s, int64_t ticket) {
   auto it = s->reqs.find(ticket);
   if (it == s->reqs.end()) return Fail(CSG_E_ARG, "csg_server_release: unknown ticket");
   if (it->second.state != kDone) return Fail(CSG_E_ARG, "csg_server_release: request not done");
-  if (it->second.mem) {
-    s->cache.insert({it->second.bytes, it->second.mem});
+  if (it->second.block.mem) {
+    s->cache.insert({it->second.bytes, it->second.block});
     s->cached += static_cast<int64_t>(it->second.bytes);
     s->used -= static_cast<int64_t>(it->second.bytes);
   }
@@ -360,7 +446,8 @@ csg_status csg_server_stats_get(csg_server* s, csg_server_stats* st) {
   if (!s || !st) return Fail(CSG_E_ARG, "csg_server_stats_get: NULL argument");
   std::lock_guard<std::mutex> lock(s->mu);
   *st = s->stats;
-  st->bytes_in_use = s->used;
+  st->bytes_in_use = s->used + s->workspace;
+  st->workspace_bytes = s->workspace;
   return CSG_OK;
 }
 
@@ -372,14 +459,17 @@ csg_status csg_server_destroy(csg_server* s) {
     s->wake.notify_one();
   }
   if (s->worker.joinable()) s->worker.join();
-  for (Slot& sl : s->slots) {
-    (void)hipStreamSynchronize(sl.stream);
-    (void)hipEventDestroy(sl.done);
-    (void)hipStreamDestroy(sl.stream);
+  (void)hipDeviceSynchronize();  // copies out of the blocks on callers' streams
+  DestroyStreams(s);
+  for (auto& kv : s->reqs) {
+    if (kv.second.block.mem) (void)hipFree(kv.second.block.mem);
+    if (kv.second.block.copied) (void)hipEventDestroy(kv.second.block.copied);
   }
-  for (auto& kv : s->reqs)
-    if (kv.second.mem) (void)hipFree(kv.second.mem);
-  for (auto& kv : s->cache) (void)hipFree(kv.second);
+  for (auto& kv : s->cache) {
+    (void)hipFree(kv.second.mem);
+    if (kv.second.copied) (void)hipEventDestroy(kv.second.copied);
+  }
+  for (hipEvent_t e : s->eventPool) (void)hipEventDestroy(e);
   delete s;
   return CSG_OK;
 }

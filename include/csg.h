@@ -96,9 +96,12 @@ typedef struct csg_scene_info {
  *   CSG_MODE_EXACT: the reference's FP32 expressions as written — IEEE
  *     division and square root, no fused multiply-add — so hit id, depth and
  *     flags are bit-identical to a host FP32 build of the reference logic.
- *   CSG_MODE_FAST: box slabs from a per-ray reciprocal direction, and fused
- *     multiply-add where the compiler finds it; decisions stay the reference's
- *     and the result is checked against the FP64 host oracle. */
+ *   CSG_MODE_FAST: box slabs as one fused multiply-add each from a per-ray
+ *     reciprocal direction, sphere and cylinder quadratics from the ray's
+ *     closest approach to the centre or axis (better conditioned than the
+ *     reference's expressions), hardware square roots and reciprocals where
+ *     they do not move a result; decisions stay the reference's and results
+ *     are checked against the FP64 host oracle (depth within 1e-4). */
 typedef enum csg_mode { CSG_MODE_EXACT = 0, CSG_MODE_FAST = 1 } csg_mode;
 
 typedef struct csg_render_opts {
@@ -265,10 +268,19 @@ csg_status csg_render_plan(const csg_scene* scene, const csg_render_opts* opts, 
 
 /* --- serving (no reference counterpart: the reference renders one frame at a
  * time for one window).  A server keeps scenes resident, queues render
- * requests, admits them in arrival order while their outputs fit a device
- * memory budget, and keeps up to `streams` batches in flight: requests that
- * arrive while others run join the next batch on a free stream
- * (continuous batching).  Outputs are server-owned until released. --- */
+ * requests, admits them in arrival order while they fit a device memory
+ * budget, and keeps up to `streams` batches in flight: requests that arrive
+ * while others run join the next batch on a free stream (continuous
+ * batching).  Outputs are server-owned until released.
+ *
+ * The budget counts all device memory the server holds: each stream's
+ * workspace (tile queue, heavy-ray list and resume area, job table; about
+ * 96 MiB a stream, allocated by csg_server_create, which fails with
+ * CSG_E_BUDGET when the budget does not cover them) and the output blocks
+ * of admitted requests until they are released.  Released blocks are reused
+ * by later requests; a block copied out with csg_server_copy is not written
+ * again before that copy has run on the caller's stream, so a request may be
+ * released right after its copy is enqueued. --- */
 typedef struct csg_server csg_server;
 typedef struct csg_server_config {
   int64_t budget_bytes;   /* device memory for request outputs; 0 = 16 GiB */
@@ -293,19 +305,23 @@ typedef struct csg_request_stats {
 } csg_request_stats;
 typedef struct csg_server_stats {
   int64_t submitted, completed, rejected, batches, rays_done;
-  int64_t bytes_in_use, peak_bytes_in_use;
+  int64_t bytes_in_use, peak_bytes_in_use; /* workspaces + admitted outputs */
+  int64_t workspace_bytes;                 /* the streams' workspaces */
 } csg_server_stats;
 csg_status csg_server_create(const csg_server_config* cfg, csg_server** out);
 /* The scene must stay alive and on the server's device while the server runs. */
 csg_status csg_server_add_scene(csg_server* server, const csg_scene* scene, int32_t* scene_id);
-/* CSG_E_BUDGET when the request's outputs alone exceed the budget. */
+/* CSG_E_BUDGET when the request's outputs exceed the budget left beside the
+ * streams' workspaces.  A request whose job fails validation when its batch
+ * forms is done at once with that status (csg_request_stats.status); one
+ * whose launch fails is done with CSG_E_HIP. */
 csg_status csg_server_submit(csg_server* server, const csg_request* req, int64_t* ticket);
 csg_status csg_server_poll(csg_server* server, int64_t ticket, int32_t* done);
 /* Blocks until the request is done; fills the server-owned device output
  * pointers (valid until release) and the request's timing. */
 csg_status csg_server_wait(csg_server* server, int64_t ticket, csg_outputs* out, csg_request_stats* stats);
 /* Copies a done request's outputs into caller-owned device buffers (any may
- * be NULL) on `stream`. */
+ * be NULL), asynchronously on `stream`; the request may be released at once. */
 csg_status csg_server_copy(csg_server* server, int64_t ticket, const csg_outputs* dst, void* stream);
 csg_status csg_server_release(csg_server* server, int64_t ticket);
 csg_status csg_server_stats_get(csg_server* server, csg_server_stats* stats);
