@@ -53,27 +53,105 @@ def _cpu_model():
     return "unknown"
 
 
-def cpu_baseline(text, cam6, light3, phong8, w, h, row_step):
-    """The oracle's FP32 restatement (the reference's arithmetic) on up to 16
-    host cores over every `row_step`-th image row, plus one core over every
-    16th row."""
+FLAG_BITS = {"tie": 1, "box": 2, "root": 4, "cyl_end": 8, "cube_normal": 16, "tangent": 32, "overflow": 64}
+
+
+def accuracy(g, o64):
+    """GPU outputs `g` against the FP64 oracle `o64` over the same rays (flat
+    arrays): hit-id agreement on all and on unflagged rays, inside/outside
+    agreement, the flagged share with each flag bit's share, and relative
+    depth error on agreeing hits (all, and unflagged)."""
+    import numpy as np
+    pid, opid = g["prim_id"].reshape(-1), o64["prim_id"].reshape(-1)
+    dg = o64["degen"].reshape(-1)
+    unf = dg == 0
+    same = pid == opid
+    hit = same & (opid >= 0)
+    rel = np.abs(g["depth"].reshape(-1)[hit].astype(np.float64) - o64["depth"].reshape(-1)[hit]) / \
+        np.abs(o64["depth"].reshape(-1)[hit].astype(np.float64))
+    rel_unf = rel[unf[hit]]
+    return {"rays": int(pid.size), "id_agree_all": round(float(same.mean()), 7),
+            "id_agree_unflagged": round(float(same[unf].mean()), 7),
+            "inside_outside_agree": round(float(((pid >= 0) == (opid >= 0)).mean()), 7),
+            "flagged": round(float(1 - unf.mean()), 6),
+            "flag_bits": {k: round(float(((dg & b) != 0).mean()), 6) for k, b in FLAG_BITS.items()},
+            "depth_rel_max": float(f"{rel.max() if rel.size else 0:.3e}"),
+            "depth_rel_p999": float(f"{np.quantile(rel, 0.999) if rel.size else 0:.3e}"),
+            "depth_rel_max_unflagged": float(f"{rel_unf.max() if rel_unf.size else 0:.3e}")}
+
+
+def cpu_baseline(cases, threads, gpu_c2=None, gpu_c5=None, mode="fast"):
+    """The oracle (the CPU restatement of the reference path, scalar C++,
+    -O2 -ffp-contract=off) timed on this host: FP32 (the reference's
+    arithmetic) and FP64, on one core and on `threads` cores, for the
+    Wikipedia 640x360 and cheese 1920x1080 frames in full and the 256-tree
+    batch and the stress frame on a 1/64 ray subsample (every 64th row).
+    The FP64 runs carry the degenerate-decision flagger and are the checker
+    of the GPU's own cheese and stress outputs (accuracy block)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
     import oracle as O
-    o = O.OracleScene(text)
-    threads = min(os.cpu_count() or 1, 16)
-    rows = len(range(0, h, row_step))
-    t0 = time.perf_counter()
-    o.render(cam6, light3, phong8, w, h, bits=32, threads=threads, rows=(0, h, row_step))
-    dt = time.perf_counter() - t0
-    rows1 = len(range(0, h, 16))
-    t1 = time.perf_counter()
-    o.render(cam6, light3, phong8, w, h, bits=32, threads=1, rows=(0, h, 16))
-    dt1 = time.perf_counter() - t1
-    return {"value": round(rows * w / dt / 1e6, 4), "unit": "Mrays/s", "cores": threads, "kind": "port",
-            "single_core_mrays_s": round(rows1 * w / dt1 / 1e6, 4), "cpu_model": _cpu_model(),
-            "host_cpus": os.cpu_count(),
-            "sample": f"every {row_step}th row of the {w}x{h} frame ({rows * w} rays), FP32 oracle, "
-                      f"{threads} threads, {dt:.1f} s; single core: every 16th row ({rows1 * w} rays), {dt1:.1f} s"}
+    legs, acc = [], {}
+
+    def leg(config, bits, nthreads, rays, sample, fn):
+        t0 = time.perf_counter()
+        res = fn(nthreads)
+        dt = time.perf_counter() - t0
+        legs.append({"config": config, "precision": f"fp{bits}", "threads": nthreads, "rays": int(rays),
+                     "seconds": round(dt, 3), "mrays_s": round(rays / dt / 1e6, 4), "sample": sample})
+        return res
+
+    c1, c2, c3, c5 = cases["c1"], cases["c2"], cases["c3"], cases["c5"]
+    # C1, the reference's own CPU configuration, in full
+    o1 = O.OracleScene(c1["text"])
+    for bits in (32, 64):
+        for n in (1, threads):
+            leg("C1 Wikipedia 640x360", bits, n, 640 * 360, "full frame",
+                lambda n, b=bits: o1.render(c1["cam6"], c1["light3"], c1["phong8"], 640, 360, bits=b, threads=n))
+    # C2, the headline frame, in full on all threads; one core on every 16th row
+    o2 = O.OracleScene(c2["text"])
+    w, h = 1920, 1080
+    leg("C2 cheese 1920x1080", 32, threads, w * h, "full frame",
+        lambda n: o2.render(c2["cam6"], c2["light3"], c2["phong8"], w, h, bits=32, threads=n))
+    r64 = leg("C2 cheese 1920x1080", 64, threads, w * h, "full frame (with the degenerate-decision flagger)",
+              lambda n: o2.render(c2["cam6"], c2["light3"], c2["phong8"], w, h, bits=64, threads=n, degen=True))
+    if gpu_c2 is not None:
+        acc[f"C2 cheese 1920x1080 {mode}, every ray"] = accuracy(gpu_c2, r64)
+    rows16 = len(range(0, h, 16)) * w
+    for bits in (32, 64):
+        leg("C2 cheese 1920x1080", bits, 1, rows16, "every 16th row",
+            lambda n, b=bits: o2.render(c2["cam6"], c2["light3"], c2["phong8"], w, h, bits=b, threads=n,
+                                        rows=(0, h, 16)))
+    # C3, the 256-tree batch, every 64th row of every job
+    scenes3 = [(O.OracleScene(t), cam6, size) for t, cam6, size in c3["jobs"]]
+    rays3 = sum(len(range(0, size, 64)) * size for _, _, size in scenes3)
+    for bits in (32, 64):
+        leg("C3 256-tree batch", bits, threads, rays3, "every 64th row of every job (1/64)",
+            lambda n, b=bits: [o.render(cam6, c3["light3"], c3["phong8"], size, size, bits=b, threads=n,
+                                        rows=(0, size, 64)) for o, cam6, size in scenes3])
+    # C5, the stress frame (4 jittered samples a pixel), every 64th row
+    o5 = O.OracleScene(c5["text"])
+    w5, h5, spp, seed = 3840, 2160, 4, c5["seed"]
+    rays5 = len(range(0, h5, 64)) * w5 * spp
+    leg("C5 stress 3840x2160x4", 32, threads, rays5, "every 64th row, all samples (1/64)",
+        lambda n: o5.render_samples(c5["cam6"], c5["light3"], c5["phong8"], w5, h5, spp, seed, bits=32, threads=n,
+                                    rows=(0, h5, 64)))
+    r5 = leg("C5 stress 3840x2160x4", 64, threads, rays5,
+             "every 64th row, all samples (1/64; with the degenerate-decision flagger)",
+             lambda n: o5.render_samples(c5["cam6"], c5["light3"], c5["phong8"], w5, h5, spp, seed, bits=64,
+                                         threads=n, rows=(0, h5, 64), degen=True))
+    if gpu_c5 is not None:
+        acc["C5 stress 3840x2160x4 fast, every 64th row"] = accuracy(
+            gpu_c5, {k: v[0:h5:64] for k, v in r5.items()})
+    headline = next(l for l in legs if l["config"].startswith("C2") and l["precision"] == "fp32"
+                    and l["threads"] == threads)
+    single = next(l for l in legs if l["config"].startswith("C2") and l["precision"] == "fp32" and l["threads"] == 1)
+    base = {"value": headline["mrays_s"], "unit": "Mrays/s", "cores": threads, "kind": "port",
+            "single_core_mrays_s": single["mrays_s"], "cpu_model": _cpu_model(), "host_cpus": os.cpu_count(),
+            "sample": f"the headline 1920x1080 frame in full ({w * h} rays), FP32 oracle, {threads} threads, "
+                      f"{headline['seconds']} s; single core: every 16th row ({rows16} rays)",
+            "legs": legs}
+    return base, acc
 
 
 def _time_ms(torch, fn, reps):
@@ -115,7 +193,8 @@ def bench_batched(torch, C, S, reps, coop):
             "rays": rays, "plain_per_scene_mrays_s": round(rays / plain_ms / 1e3, 1),
             "batched_mrays_s": round(rays / batch_ms / 1e3, 1), "graph_mrays_s": round(rays / graph_ms / 1e3, 1),
             "batched_vs_plain": round(plain_ms / min(batch_ms, graph_ms), 2),
-            "autotune": {"block": tune.block, "tile_walk": tune.tile_walk, "heaviest_first": tune.heaviest_first},
+            "autotune": {"block": tune.block, "tile_walk": tune.tile_walk, "heaviest_first": tune.heaviest_first,
+                         "depth_split": tune.depth_split},
             "ms": {"plain": round(plain_ms, 2), "batched": round(batch_ms, 2), "graph": round(graph_ms, 2)}}
 
 
@@ -157,6 +236,8 @@ def bench_stress(torch, C, S, reps, coop):
     graph_ms = _time_ms(torch, batch.launch, reps)
     same = all(np.array_equal(plain_img[k].view(np.uint8), bouts[k].cpu().numpy().view(np.uint8))
                for k in ("rgba", "depth", "prim_id", "flags"))
+    rows64 = {k: plain_img[k][0:h:64].copy() for k in ("depth", "prim_id", "flags")}
+    del plain_img
     batch.close()
     rays = w * h * spp
     info = scene.info()
@@ -165,8 +246,9 @@ def bench_stress(torch, C, S, reps, coop):
             "rays": rays, "nodes": info.nodes, "op_depth": info.op_depth, "scene_in_lds": bool(plan.scene_in_lds),
             "plain_ms": round(plain_ms, 2), "plain_mrays_s": round(rays / plain_ms / 1e3, 1),
             "graph_ms": round(graph_ms, 2), "graph_mrays_s": round(rays / graph_ms / 1e3, 1),
-            "autotune": {"block": tune.block, "tile_walk": tune.tile_walk, "heaviest_first": tune.heaviest_first},
-            "coop_rays_per_frame": round(handed), "plain_equals_batched": same}
+            "autotune": {"block": tune.block, "tile_walk": tune.tile_walk, "heaviest_first": tune.heaviest_first,
+                         "depth_split": tune.depth_split},
+            "coop_rays_per_frame": round(handed), "plain_equals_batched": same, "_gpu_rows64": rows64}
 
 
 def bench_serving(C, S, lambdas, seconds, coop):
@@ -257,7 +339,6 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--mode", choices=("fast", "exact"), default="fast")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-row-step", type=int, default=1)
     ap.add_argument("--no-extra", action="store_true", help="skip the batched and serving legs")
     ap.add_argument("--serve-seconds", type=float, default=3.0)
     ap.add_argument("--coop-budget", type=int, default=0,
@@ -350,10 +431,23 @@ def main():
         if args.only in (None, "serving"):
             extra["serving"] = bench_serving(C, S, (200, 1000, 5000), args.serve_seconds, coop)
 
-    cpu = None
+    gpu_c5 = extra.get("stress", {}).pop("_gpu_rows64", None)
+    cpu, acc = None, None
     if not args.no_cpu:
-        cpu = cpu_baseline(text, C.cam_tuple(cam), S.LIGHT_TOWARD, S.PHONG + S.BACKGROUND, w, h,
-                           args.cpu_row_step)
+        import numpy as np  # noqa: F401
+        gpu_c2 = {k: outs[k].cpu().numpy() for k in ("depth", "prim_id", "flags")}
+        ph8 = S.PHONG + S.BACKGROUND
+        view6 = C.cam_tuple(cam)
+        cases = {
+            "c1": {"text": S.WIKIPEDIA, "cam6": view6, "light3": S.LIGHT_TOWARD, "phong8": ph8},
+            "c2": {"text": text, "cam6": view6, "light3": S.LIGHT_TOWARD, "phong8": ph8},
+            "c3": {"jobs": [(t, C.cam_tuple(C.camera_look_at(eye, (0, 0, 0), 60)), size)
+                            for t, eye, size in S.batch_mix(256, seed=256)],
+                   "light3": S.LIGHT_TOWARD, "phong8": ph8},
+            "c5": {"text": S.stress(4095, seed=4095), "cam6": view6, "light3": S.LIGHT_TOWARD, "phong8": ph8,
+                   "seed": 4095},
+        }
+        cpu, acc = cpu_baseline(cases, min(os.cpu_count() or 1, 16), gpu_c2, gpu_c5, args.mode)
 
     line = {
         "metric": "Mrays/s shaded primary rays (plain vs batched) + p50/p99 frame latency",
@@ -369,6 +463,7 @@ def main():
                    "coop_grid": plan.coop_grid, "coop_block": plan.coop_block},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "accuracy": acc,
     }
     line.update(extra)
     print(json.dumps(line))

@@ -21,7 +21,7 @@ OK, E_PARSE, E_DEPTH, E_OOM, E_HIP, E_ARG, E_BUDGET = range(7)
 STATUS_NAMES = {0: "CSG_OK", 1: "CSG_E_PARSE", 2: "CSG_E_DEPTH", 3: "CSG_E_OOM", 4: "CSG_E_HIP",
                 5: "CSG_E_ARG", 6: "CSG_E_BUDGET"}
 MODE_EXACT, MODE_FAST = 0, 1
-PLACE_AUTO, PLACE_GLOBAL, PLACE_LDS = 0, 1, 2
+PLACE_AUTO, PLACE_GLOBAL, PLACE_LDS, PLACE_CACHE = 0, 1, 2, 3
 
 # hit flag bits (final record)
 ENTER, EXIT, MISS, FLIP, CAP_BOTTOM, CAP_TOP = 2, 4, 8, 16, 32, 64
@@ -68,12 +68,12 @@ class LaunchInfo(ctypes.Structure):
     _fields_ = [("block", ctypes.c_int32), ("grid", ctypes.c_int32), ("lds_bytes", ctypes.c_int64),
                 ("scene_in_lds", ctypes.c_int32), ("compute_units", ctypes.c_int32), ("lds_limit", ctypes.c_int64),
                 ("coop_budget", ctypes.c_int32), ("coop_grid", ctypes.c_int32), ("coop_block", ctypes.c_int32),
-                ("coop_grace", ctypes.c_int32), ("coop_lds_bytes", ctypes.c_int64)]
+                ("coop_grace", ctypes.c_int32), ("coop_lds_bytes", ctypes.c_int64), ("cache_ops", ctypes.c_int32)]
 
 
 class BatchTuning(ctypes.Structure):
     _fields_ = [("block", ctypes.c_int32), ("tile_walk", ctypes.c_int32), ("heaviest_first", ctypes.c_int32),
-                ("ms", ctypes.c_float)]
+                ("ms", ctypes.c_float), ("depth_split", ctypes.c_int32)]
 
 
 class ServerConfig(ctypes.Structure):

@@ -5,6 +5,13 @@
 // once into a hipGraph.  An update rewrites the job tables in stream order and
 // patches the kernel nodes whose launch parameters changed; a launch replays
 // the graph.
+//
+// Depth classes: with a split depth D > 0 the jobs whose trees have at most D
+// operator levels and the deeper ones go to separate launches, so that each
+// launch's LDS stacks hold only as many slots as its own deepest tree needs
+// (one 8-byte slot per level and lane) and the shallow trees' workgroups are
+// not sized for the deep ones.  The autotuner tries no split and one split
+// (HalfWorkDepth).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -39,6 +46,7 @@ struct csg_batch {
   csg_render_opts opts{};
   csgray::LaunchConfig cfg;
   bool heaviestFirst = false;
+  int32_t depthSplit = 0;  // 0: one class; else jobs of depth <= depthSplit launch apart from deeper ones
   std::vector<csg_job> jobs;
   std::vector<Chunk> chunks;
   hipGraph_t graph = nullptr;
@@ -76,28 +84,110 @@ void DestroyGraph(csg_batch* b) {
   b->graph = nullptr;
 }
 
-// Fills each chunk's staging table and launch parameters from b->jobs in the
-// batch's job order.  Does not touch device memory.
-csg_status Stage(csg_batch* b) {
+// The batch's launches: job indices in launch order, and for each launch
+// (chunk) its first position in that order and its job count.  Jobs go in
+// the given order or heaviest first, then (with a depth split) the shallow
+// class before the deep one; each class is cut into chunks of at most
+// kMaxBatchJobs jobs.
+struct Plan {
+  std::vector<int32_t> order;
+  std::vector<std::pair<int32_t, int32_t>> chunks;  // (start, count)
+};
+
+csg_status MakePlan(const csg_batch* b, Plan* plan) {
   const int32_t n = static_cast<int32_t>(b->jobs.size());
-  std::vector<int32_t> order(static_cast<size_t>(n));
-  for (int32_t i = 0; i < n; i++) order[i] = i;
+  plan->order.resize(static_cast<size_t>(n));
+  for (int32_t i = 0; i < n; i++) plan->order[i] = i;
   if (b->heaviestFirst) {
-    const csg_status st = csg_batch_order(b->jobs.data(), n, order.data());
+    const csg_status st = csg_batch_order(b->jobs.data(), n, plan->order.data());
     if (st != CSG_OK) return st;
   }
-  const int32_t need = (n + csgray::kMaxBatchJobs - 1) / csgray::kMaxBatchJobs;
-  if (need > static_cast<int32_t>(b->chunks.size()))
+  int32_t shallow = n;
+  if (b->depthSplit > 0) {
+    auto mid = std::stable_partition(plan->order.begin(), plan->order.end(), [&](int32_t i) {
+      return b->jobs[i].scene && b->jobs[i].scene->dev.opDepth <= b->depthSplit;
+    });
+    shallow = static_cast<int32_t>(mid - plan->order.begin());
+  }
+  plan->chunks.clear();
+  for (const auto& cls : {std::make_pair(0, shallow), std::make_pair(shallow, n)})
+    for (int32_t s = cls.first; s < cls.second; s += csgray::kMaxBatchJobs)
+      plan->chunks.push_back({s, std::min(cls.second - s, csgray::kMaxBatchJobs)});
+  return CSG_OK;
+}
+
+// Allocates chunks until there are as many as the batch's launch plan has.
+csg_status EnsureChunks(csg_batch* b) {
+  Plan plan;
+  const csg_status st = MakePlan(b, &plan);
+  if (st != CSG_OK) return st;
+  while (b->chunks.size() < plan.chunks.size()) {
+    b->chunks.emplace_back();
+    const csg_status a = AllocChunk(b->chunks.back(), csgray::kMaxBatchJobs);
+    if (a != CSG_OK) return a;
+  }
+  return CSG_OK;
+}
+
+// Frees the chunks beyond the launch plan (the device is idle).
+void TrimChunks(csg_batch* b) {
+  Plan plan;
+  if (MakePlan(b, &plan) != CSG_OK) return;
+  const size_t keep = std::max<size_t>(plan.chunks.size(), 1);
+  while (b->chunks.size() > keep) {
+    Chunk& c = b->chunks.back();
+    if (c.devJobs) (void)hipFree(c.devJobs);
+    if (c.hostJobs) (void)hipHostFree(c.hostJobs);
+    if (c.queue) (void)hipFree(c.queue);
+    if (c.heavy) (void)hipFree(c.heavy);
+    b->chunks.pop_back();
+  }
+}
+
+// The split depth the autotuner tries: the smallest d with at least half of
+// the batch's work (pixels x (1 + depth)) in trees of depth <= d, or, when
+// that is the deepest tree, the largest depth below it; 0 when all trees
+// have one depth (no second class).
+int32_t HalfWorkDepth(const csg_batch* b) {
+  std::vector<std::pair<int32_t, double>> w;
+  double total = 0;
+  for (const csg_job& j : b->jobs) {
+    const int32_t d = j.scene ? j.scene->dev.opDepth : 0;
+    const double work = static_cast<double>(j.width) * j.height * (1.0 + d);
+    w.push_back({d, work});
+    total += work;
+  }
+  std::sort(w.begin(), w.end());
+  if (w.empty() || w.front().first == w.back().first) return 0;
+  double acc = 0;
+  for (const auto& x : w) {
+    acc += x.second;
+    if (acc >= 0.5 * total && x.first < w.back().first) return x.first;
+  }
+  int32_t below = 0;
+  for (const auto& x : w)
+    if (x.first < w.back().first) below = x.first;
+  return below;
+}
+
+// Fills each chunk's staging table and launch parameters from b->jobs in the
+// batch's launch plan (chunks beyond the plan get no jobs).  Does not touch
+// device memory.
+csg_status Stage(csg_batch* b) {
+  Plan plan;
+  const csg_status pst = MakePlan(b, &plan);
+  if (pst != CSG_OK) return pst;
+  if (plan.chunks.size() > b->chunks.size())
     return Fail(CSG_E_ARG, "csg_batch: more jobs than the batch was created for");
   for (int32_t c = 0; c < static_cast<int32_t>(b->chunks.size()); c++) {
     Chunk& ch = b->chunks[c];
-    const int32_t start = c * csgray::kMaxBatchJobs;
-    const int32_t cnt = std::max(0, std::min(n - start, csgray::kMaxBatchJobs));
+    const int32_t start = c < static_cast<int32_t>(plan.chunks.size()) ? plan.chunks[c].first : 0;
+    const int32_t cnt = c < static_cast<int32_t>(plan.chunks.size()) ? plan.chunks[c].second : 0;
     if (cnt > ch.cap) return Fail(CSG_E_ARG, "csg_batch: chunk capacity exceeded");
     int64_t base = 0;
     int32_t depth = 1, maxOps = 1, maxLeaves = 1;
     for (int32_t i = 0; i < cnt; i++) {
-      const csg_job& j = b->jobs[order[start + i]];
+      const csg_job& j = b->jobs[plan.order[start + i]];
       maxOps = std::max(maxOps, j.scene->dev.nOps);
       maxLeaves = std::max(maxLeaves, j.scene->dev.opLeaves);
       const csg_status st = BuildJob(j.scene, &j.camera, &j.light, &j.phong, j.width, j.height, &j.out, &b->opts,
@@ -293,7 +383,8 @@ csg_status csg_batch_update(csg_batch* b, const csg_job* jobs, int32_t njobs, vo
   if (st != CSG_OK) return st;
   std::vector<csg_job> keep = b->jobs;
   b->jobs.assign(jobs, jobs + njobs);
-  st = Stage(b);
+  st = EnsureChunks(b);
+  if (st == CSG_OK) st = Stage(b);
   if (st != CSG_OK) {
     b->jobs = keep;
     (void)Stage(b);
@@ -320,10 +411,15 @@ csg_status csg_batch_autotune(csg_batch* b, int32_t reps, void* stream, csg_batc
     return Fail(CSG_E_HIP, "hipEventCreate failed");
   const csg_render_opts base = b->opts;
   const bool baseHeavy = b->heaviestFirst;
+  const int32_t baseSplit = b->depthSplit;
   float best = 1e30f;
   csg_render_opts bestOpts = base;
   bool bestHeavy = baseHeavy;
+  int32_t bestSplit = baseSplit;
   csg_status st = CSG_OK;
+  std::vector<int32_t> splits{0};
+  if (const int32_t half = HalfWorkDepth(b)) splits.push_back(half);
+  for (int32_t split : splits)
   for (int32_t block : {256, 512, 1024}) {
     for (int32_t walk : {0, 1}) {
       for (bool heavy : {false, true}) {
@@ -332,8 +428,10 @@ csg_status csg_batch_autotune(csg_batch* b, int32_t reps, void* stream, csg_batc
         o.tile_walk = walk;
         b->opts = o;
         b->heaviestFirst = heavy;
+        b->depthSplit = split;
         st = ParseOpts(&b->opts, &b->cfg);
         if (st == CSG_OK) st = WaitStaged(b);
+        if (st == CSG_OK) st = EnsureChunks(b);
         if (st == CSG_OK) st = Stage(b);
         if (st == CSG_OK) st = Upload(b, hs);
         if (st == CSG_OK) st = LaunchDirect(b, hs);  // warm-up
@@ -350,6 +448,7 @@ csg_status csg_batch_autotune(csg_batch* b, int32_t reps, void* stream, csg_batc
           best = ms;
           bestOpts = o;
           bestHeavy = heavy;
+          bestSplit = split;
         }
       }
       if (st != CSG_OK) break;
@@ -360,8 +459,14 @@ csg_status csg_batch_autotune(csg_batch* b, int32_t reps, void* stream, csg_batc
   (void)hipEventDestroy(t1);
   b->opts = st == CSG_OK ? bestOpts : base;
   b->heaviestFirst = st == CSG_OK ? bestHeavy : baseHeavy;
+  b->depthSplit = st == CSG_OK ? bestSplit : baseSplit;
   csg_status st2 = ParseOpts(&b->opts, &b->cfg);
   if (st2 == CSG_OK) st2 = WaitStaged(b);
+  if (st2 == CSG_OK && hipStreamSynchronize(hs) != hipSuccess) st2 = Fail(CSG_E_HIP, "hipStreamSynchronize failed");
+  if (st2 == CSG_OK) {
+    DestroyGraph(b);  // the graph may point at chunks about to be freed
+    TrimChunks(b);
+  }
   if (st2 == CSG_OK) st2 = Stage(b);
   if (st2 == CSG_OK) st2 = Upload(b, hs);
   if (st2 == CSG_OK && hipStreamSynchronize(hs) != hipSuccess) st2 = Fail(CSG_E_HIP, "hipStreamSynchronize failed");
@@ -373,6 +478,7 @@ csg_status csg_batch_autotune(csg_batch* b, int32_t reps, void* stream, csg_batc
     chosen->tile_walk = b->opts.tile_walk;
     chosen->heaviest_first = b->heaviestFirst ? 1 : 0;
     chosen->ms = best;
+    chosen->depth_split = b->depthSplit;
   }
   return CSG_OK;
 }

@@ -138,9 +138,8 @@ CoopPlan PlanCoop(const LaunchConfig& cfg, int32_t maxOps, int32_t maxLeaves) {
   const size_t wave = CoopLdsBytes(maxOps, maxLeaves, &c.queueCap);
   if (wave == 0) return c;
   int cus = 0;
-  size_t ldsPerBlock = 0;
-  if (DeviceShapeLimits(&cus, &ldsPerBlock) != hipSuccess || cus <= 0) return c;
-  const size_t ldsPerCu = 160 * 1024;
+  size_t ldsPerBlock = 0, ldsPerCu = 0;
+  if (DeviceShapeLimits(&cus, &ldsPerBlock, &ldsPerCu) != hipSuccess || cus <= 0) return c;
   const size_t perCu = std::max<size_t>(1, std::min<size_t>(16, ldsPerCu / (wave + 256)));
   c.budget = cfg.budget > 0 ? cfg.budget : kDefaultBudget;
   c.waves = static_cast<uint32_t>(static_cast<size_t>(cus) * perCu);
@@ -177,6 +176,7 @@ void PrepareCoop(const LaunchConfig& cfg, int32_t maxOps, int32_t maxLeaves, uin
   p->heavy = heavy;
   p->heavyCap = kHeavyCap;
   p->resume = reinterpret_cast<uint32_t*>(heavy + kHeavyCap);
+  p->resumeBytes = static_cast<uint32_t>(kResumeBytes);
   p->coopOps = maxOps;
   p->coopQueueCap = c.queueCap;
   p->coopWaves = c.waves;
@@ -385,12 +385,14 @@ csg_status ParseOpts(const csg_render_opts* opts, csgray::LaunchConfig* cfg) {
   if (cfg->count && !opts->counter_out) return Fail(CSG_E_ARG, "csg_render: counters requested without a buffer");
   if (opts->block != 0 && opts->block != 256 && opts->block != 512 && opts->block != 1024)
     return Fail(CSG_E_ARG, "csg_render: block must be 0, 256, 512 or 1024");
-  if (opts->scene_placement < 0 || opts->scene_placement > 2)
-    return Fail(CSG_E_ARG, "csg_render: scene_placement must be 0, 1 or 2");
+  if (opts->scene_placement < 0 || opts->scene_placement > 3)
+    return Fail(CSG_E_ARG, "csg_render: scene_placement must be 0, 1, 2 or 3");
   if (opts->samples < 0 || opts->samples > 1024) return Fail(CSG_E_ARG, "csg_render: samples must be in [0, 1024]");
   if (opts->tile_walk != 0 && opts->tile_walk != 1) return Fail(CSG_E_ARG, "csg_render: tile_walk must be 0 or 1");
   cfg->block = opts->block;
-  cfg->sceneInLds = opts->scene_placement == 0 ? -1 : (opts->scene_placement == 2 ? 1 : 0);
+  cfg->sceneInLds = opts->scene_placement == 0 ? -1
+                    : opts->scene_placement == 2 ? 1
+                    : opts->scene_placement == 3 ? 2 : 0;
   cfg->morton = opts->tile_walk == 1;
   cfg->budget = opts->coop_budget;
   if (opts->coop_when != 0 && opts->coop_when != 1) return Fail(CSG_E_ARG, "coop_when must be 0 or 1");
@@ -565,6 +567,7 @@ csg_status csg_render_plan(const csg_scene* s, const csg_render_opts* opts, csg_
   info->grid = shape.grid;
   info->lds_bytes = static_cast<int64_t>(shape.lds);
   info->scene_in_lds = shape.sceneInLds ? 1 : 0;
+  info->cache_ops = shape.cacheOps;
   info->compute_units = cus;
   info->lds_limit = static_cast<int64_t>(ldsLimit);
   const CoopPlan c = PlanCoop(cfg, s->dev.nOps, s->dev.opLeaves);
@@ -627,6 +630,7 @@ csg_status csg_render_ex(const csg_scene* s, const csg_camera* cam, const csg_li
   if (e != hipSuccess) return FailHip(e, "choosing the launch shape");
   csgray::FitGridToTiles(&shape, p.numTiles);
   p.totalWaves = static_cast<uint32_t>(shape.grid) * static_cast<uint32_t>(shape.block / 64);
+  p.cacheOps = shape.cacheOps;
   const hipStream_t hs = static_cast<hipStream_t>(stream);
   const csg_status qs = QueueFor(dev, hs, &p.queue);
   if (qs != CSG_OK) return qs;
@@ -664,7 +668,7 @@ csg_status csg_render_batch(const csg_job* jobs, int32_t njobs, const csg_render
   csgray::LaunchConfig cfg;
   const csg_status ost = ParseOpts(opts, &cfg);
   if (ost != CSG_OK) return ost;
-  if (cfg.sceneInLds == 1) return Fail(CSG_E_ARG, "csg_render_batch: scenes of a batch are read from global memory");
+  if (cfg.sceneInLds >= 1) return Fail(CSG_E_ARG, "csg_render_batch: scenes of a batch are read from global memory");
   cfg.sceneInLds = 0;
   int dev = 0;
   hipError_t e = hipGetDevice(&dev);

@@ -69,10 +69,12 @@ struct RenderParams {
   int32_t coopOps;            // operator-state slots per wave (max operators of the launch)
   uint32_t coopQueueCap;      // task ring entries (power of two >= the launch's largest opLeaves)
   uint32_t coopWaves;         // waves in the cooperative grid
-  // Resume area (kResumeBytes, right after the heavy list): for the first
+  // Resume area (`resumeBytes`, right after the heavy list): for the first
   // ResumeCap() handed-over rays, the state machine's state at hand-over, so
   // the cooperative kernel continues the ray instead of restarting it.
   uint32_t* resume;
+  uint32_t resumeBytes;
+  int32_t cacheOps;           // plain global-memory launches: operator records [0, cacheOps) staged in LDS
 };
 // queue words: [1] render workgroups done, [2] heavy rays, [4] coop waves done, [6..7] heavy rays handed
 // over on this stream so far (64-bit); on their own cache line, [32..33] time the first tile was taken,
@@ -105,7 +107,7 @@ struct LaunchConfig {
   Mode mode = Mode::kFast;
   bool count = false;
   int32_t block = 0;   // 0 = choose
-  int32_t sceneInLds = -1;  // -1 = choose, 0 = global, 1 = LDS
+  int32_t sceneInLds = -1;  // -1 = choose, 0 = global, 1 = LDS, 2 = global with the top of the tree cached in LDS
   bool morton = false;      // visit each job's tiles in Morton order
   int32_t budget = 0;       // 0: default step budget, < 0: no deferral, > 0: steps before deferral
   bool coopEager = false;   // hand rays over as soon as the budget is spent, not only after the queue drains
@@ -118,6 +120,7 @@ struct LaunchShape {
   int32_t grid = 0;
   size_t lds = 0;
   bool sceneInLds = false;
+  int32_t cacheOps = 0;  // operator records cached in LDS (scene in global memory)
 };
 
 // Fewer workgroups when the launch has fewer tiles than the grid has waves:
@@ -136,8 +139,8 @@ inline void FitGridToTiles(LaunchShape* s, int64_t tiles) {
 hipError_t ChooseShape(int32_t depth, size_t sceneBytes, const LaunchConfig& cfg, LaunchShape* shape,
                        int32_t batchJobs = 0, int64_t tiles = 0, int32_t ops = 0);
 
-// Compute units and LDS bytes per workgroup of the current device.
-hipError_t DeviceShapeLimits(int* computeUnits, size_t* ldsPerBlock);
+// Compute units, LDS bytes per workgroup and per CU of the current device.
+hipError_t DeviceShapeLimits(int* computeUnits, size_t* ldsPerBlock, size_t* ldsPerCu = nullptr);
 
 // Enqueues one launch on `stream` with the given shape (p.numJobs == 0:
 // plain render of p.job; otherwise the batch in p.jobs).

@@ -38,6 +38,9 @@
 
 #include <algorithm>
 #include <cstdint>
+#include <map>
+#include <mutex>
+#include <utility>
 
 #include "device.hpp"
 
@@ -388,12 +391,22 @@ struct Counts {
   int32_t go, comp, box, sph, cyl, cub;
 };
 
-// Scene arrays, either LDS copies or the global originals.
+// Scene arrays, either LDS copies or the global originals.  With an LDS
+// cache of the top of the tree (kCache variants), `ops` holds the first
+// `nCached` operator records in LDS and `gops` all of them in global memory.
 struct SceneView {
   const float4* ops;
   const float4* primA;
   const float4* primB;
+  const float4* gops;
+  uint32_t nCached;
 };
+
+// The array holding operator k's record: the LDS cache when k is cached.
+template <bool kCache>
+__device__ __forceinline__ const float4* OpBase(const SceneView& sv, uint32_t k) {
+  return (kCache && k >= sv.nCached) ? sv.gops : sv.ops;
+}
 
 // hitPrimitive (:113-133) for a primitive descriptor.  `h` is the record the
 // result replaces (its stale fields survive a miss, as in the reference).
@@ -450,7 +463,7 @@ struct TraceState {
   Hit L, R, held;
 };
 
-template <bool kFast, bool kCount>
+template <bool kFast, bool kCount, bool kCache = false>
 __device__ Hit Trace(const SceneView& sv, const RayF& r, uint32_t rootDesc, uint2* stk, int stride, int cap,
                      const uint16_t* table, Counts& n, int budget, const uint64_t* lstamps,
                      const uint64_t* gstamps, int grace, bool& exceeded, TraceState* ts = nullptr) {
@@ -500,10 +513,11 @@ __device__ Hit Trace(const SceneView& sv, const RayF& r, uint32_t rootDesc, uint
       } else {
         const uint32_t k = DescIndex(d);
         cur = k;
-        const float4 q0 = sv.ops[4 * k + 0];
-        const float4 q1 = sv.ops[4 * k + 1];
-        const float4 q2 = sv.ops[4 * k + 2];
-        const float4 q3 = sv.ops[4 * k + 3];
+        const float4* ob = OpBase<kCache>(sv, k);
+        const float4 q0 = ob[4 * k + 0];
+        const float4 q1 = ob[4 * k + 1];
+        const float4 q2 = ob[4 * k + 2];
+        const float4 q3 = ob[4 * k + 3];
         lDesc = AsU(q3.x);
         rDesc = AsU(q3.y);
         opType = AsU(q3.z) & 0xFFu;
@@ -556,7 +570,8 @@ __device__ Hit Trace(const SceneView& sv, const RayF& r, uint32_t rootDesc, uint
       const uint32_t dec = Decide(table, opType, L, R);
       const int top = min(max(sp - stride, 0), lastE);
       const uint2 e = stk[top];
-      const float4 pq = sv.ops[4 * (opParent == kNone ? 0u : opParent) + 3];
+      const uint32_t pk = opParent == kNone ? 0u : opParent;
+      const float4 pq = OpBase<kCache>(sv, pk)[4 * pk + 3];
       // Keep the three reads ahead of the branch below (one LDS round trip):
       // left alone, the compiler sinks the stack and parent reads into the
       // return path, after the decision's read has completed.
@@ -782,7 +797,7 @@ __device__ __forceinline__ uint32_t Compact1By1(uint32_t v) {
 // of kResumeWords words, then ResumeCap x cap stack entries, then as many
 // owners (the operator each stack slot belongs to).
 __device__ __forceinline__ uint32_t ResumeCap(const RenderParams& p, int cap) {
-  const size_t n = kResumeBytes / (kResumeWords * 4 + 12 * static_cast<size_t>(cap));
+  const size_t n = p.resumeBytes / (kResumeWords * 4 + 12 * static_cast<size_t>(cap));
   return n < p.heavyCap ? static_cast<uint32_t>(n) : p.heavyCap;
 }
 __device__ __forceinline__ uint2* ResumeEntries(const RenderParams& p, uint32_t rcap) {
@@ -793,7 +808,7 @@ __device__ __forceinline__ uint32_t* ResumeOwners(const RenderParams& p, uint32_
 }
 
 // Traces and shades the 64 rays of one tile (one per lane).
-template <bool kFast, bool kCount>
+template <bool kFast, bool kCount, bool kCache = false>
 __device__ __forceinline__ void RenderTile(const Job& jb, const SceneView& sv, uint32_t local, bool morton,
                                            uint2* slab, int stride, int cap, const uint16_t* table, int lane,
                                            const RenderParams& p, uint32_t jobIndex, const uint64_t* lstamps) {
@@ -825,8 +840,8 @@ __device__ __forceinline__ void RenderTile(const Job& jb, const SceneView& sv, u
   const int budget = p.budget > 0 ? p.budget : 0x7FFFFFFF;
   const uint64_t* gstamps = p.coopEager ? nullptr : reinterpret_cast<const uint64_t*>(p.queue + 32);
   TraceState ts;
-  Hit h = Trace<kFast, kCount>(sv, r, jb.sc.rootDesc, slab, stride, cap, table, n, budget, lstamps, gstamps,
-                               p.coopGrace, exceeded, &ts);
+  Hit h = Trace<kFast, kCount, kCache>(sv, r, jb.sc.rootDesc, slab, stride, cap, table, n, budget, lstamps,
+                                       gstamps, p.coopGrace, exceeded, &ts);
   if (exceeded) {
     // Hand the ray to the cooperative kernel; if its list is full, finish
     // here.  The first ResumeCap rays leave their state, so the cooperative
@@ -856,18 +871,19 @@ __device__ __forceinline__ void RenderTile(const Job& jb, const SceneView& sv, u
         // One slot per operator on the path from the root: the top one is
         // the current operator's when it is about to enter a child, else
         // its parent's; each lower one belongs to the next ancestor.
-        uint32_t a = ts.action <= static_cast<uint32_t>(kGotoRgh) ? ts.cur : AsU(sv.ops[4 * ts.cur + 3].w);
+        uint32_t a = ts.action <= static_cast<uint32_t>(kGotoRgh) ? ts.cur
+                                                                   : AsU(OpBase<kCache>(sv, ts.cur)[4 * ts.cur + 3].w);
         for (int i = ts.slots - 1; i >= 0; i--) {
           ent[i] = slab[i * stride];
           own[i] = a;
-          if (i > 0) a = a == kNone ? kNone : AsU(sv.ops[4 * a + 3].w);
+          if (i > 0) a = a == kNone ? kNone : AsU(OpBase<kCache>(sv, a)[4 * a + 3].w);
         }
       }
       return;
     }
     n = Counts{0, 0, 0, 0, 0, 0};
-    h = Trace<kFast, kCount>(sv, r, jb.sc.rootDesc, slab, stride, cap, table, n, 0x7FFFFFFF, lstamps, nullptr, 0,
-                             exceeded);
+    h = Trace<kFast, kCount, kCache>(sv, r, jb.sc.rootDesc, slab, stride, cap, table, n, 0x7FFFFFFF, lstamps,
+                                     nullptr, 0, exceeded);
   }
   ShadeAndStore<kFast>(jb, sv, r, h, out);
   if (kCount && jb.counters) {
@@ -897,7 +913,7 @@ __device__ __forceinline__ uint32_t NextStripe(uint64_t m, uint32_t stripe) {
   return static_cast<uint32_t>(__ffsll(static_cast<unsigned long long>(after != 0 ? after : m)) - 1);
 }
 
-template <bool kFast, bool kCount, bool kLds, bool kBatch, bool kXcd>
+template <bool kFast, bool kCount, bool kLds, bool kBatch, bool kXcd, bool kCache = false>
 __device__ __forceinline__ void RenderBody(const RenderParams& p) {
   extern __shared__ uint4 ldsBase[];
   __shared__ uint16_t table[kDecisionEntries];
@@ -909,7 +925,15 @@ __device__ __forceinline__ void RenderBody(const RenderParams& p) {
   if (tid < kDecisionEntries) table[tid] = kDecisions.v[tid];
   int32_t* jobBase = reinterpret_cast<int32_t*>(slab + static_cast<size_t>(cap) * blockDim.x);
 
-  SceneView sv{p.job.sc.ops, p.job.sc.primA, p.job.sc.primB};
+  SceneView sv{p.job.sc.ops, p.job.sc.primA, p.job.sc.primB, p.job.sc.ops, 0u};
+  if (kCache) {
+    // the top of the tree: operator records [0, cacheOps) (level order)
+    float4* lds = reinterpret_cast<float4*>(slab + static_cast<size_t>(cap) * blockDim.x);
+    const int n4 = 4 * p.cacheOps;
+    for (int i = tid; i < n4; i += blockDim.x) lds[i] = p.job.sc.ops[i];
+    sv.ops = lds;
+    sv.nCached = static_cast<uint32_t>(p.cacheOps);
+  }
   if (kLds) {
     float4* lds = reinterpret_cast<float4*>(slab + static_cast<size_t>(cap) * blockDim.x);
     const int nOps4 = 4 * p.job.sc.nOps, nP = p.job.sc.nPrims;
@@ -983,13 +1007,13 @@ __device__ __forceinline__ void RenderBody(const RenderParams& p) {
       }
       const int jj = __builtin_amdgcn_readfirstlane(lo);
       const Job& jb = p.jobs[jj];
-      const SceneView jsv{jb.sc.ops, jb.sc.primA, jb.sc.primB};
+      const SceneView jsv{jb.sc.ops, jb.sc.primA, jb.sc.primB, jb.sc.ops, 0u};
       RenderTile<kFast, kCount>(jb, jsv, q - static_cast<uint32_t>(jb.tileBase), p.morton != 0, slab + tid,
                                 blockDim.x, cap, table, lane, p, static_cast<uint32_t>(jj), ldsStamps);
     } else {
       const uint32_t t = p.tileOrder ? p.tileOrder[q] : q;
-      RenderTile<kFast, kCount>(p.job, sv, t, p.morton != 0, slab + tid, blockDim.x, cap, table, lane, p, 0u,
-                                ldsStamps);
+      RenderTile<kFast, kCount, kCache>(p.job, sv, t, p.morton != 0, slab + tid, blockDim.x, cap, table, lane, p,
+                                        0u, ldsStamps);
     }
   }
   // The last workgroup out returns the tile counters and the time stamps to
@@ -1012,9 +1036,9 @@ __device__ __forceinline__ void RenderBody(const RenderParams& p) {
 // tile phase of short-ray scenes measured 20-32 % faster; 1024-lane launches
 // of long-ray scenes keep the global mapping, whose tail hand-over grace is
 // tuned to its tile phase).
-template <bool kFast, bool kCount, bool kLds, bool kXcd>
+template <bool kFast, bool kCount, bool kLds, bool kXcd, bool kCache = false>
 __global__ void __launch_bounds__(1024) RenderKernel(RenderParams p) {
-  RenderBody<kFast, kCount, kLds, false, kXcd>(p);
+  RenderBody<kFast, kCount, kLds, false, kXcd, kCache>(p);
 }
 
 // Batched launches: many scenes in global memory, latency-bound on L2 reads,
@@ -1023,8 +1047,12 @@ __global__ void __launch_bounds__(1024) RenderKernel(RenderParams p) {
 // SIMD (a few spilled bytes): measured 7% faster than the unconstrained 90
 // registers on the 256-tree batch; the plain global-memory launch measured
 // slower with such a budget.
+#ifndef CSG_BATCH_WAVES_PER_SIMD
+#define CSG_BATCH_WAVES_PER_SIMD 7
+#endif
 template <bool kFast, bool kCount>
-__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(7))) BatchRenderKernel(RenderParams p) {
+__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(CSG_BATCH_WAVES_PER_SIMD)))
+BatchRenderKernel(RenderParams p) {
   RenderBody<kFast, kCount, false, true, false>(p);
 }
 
@@ -1370,7 +1398,7 @@ __global__ void __launch_bounds__(64) CoopKernel(RenderParams p) {
     const uint2 e = p.heavy[k];
     const uint32_t jobWord = __builtin_amdgcn_readfirstlane(e.x);
     const Job& jb = kBatch ? p.jobs[jobWord & ~kResumeFlag] : p.job;
-    const SceneView sv{jb.sc.ops, jb.sc.primA, jb.sc.primB};
+    const SceneView sv{jb.sc.ops, jb.sc.primA, jb.sc.primB, jb.sc.ops, 0u};
     const uint32_t out = __builtin_amdgcn_readfirstlane(e.y);
     const uint32_t spp = static_cast<uint32_t>(jb.spp);
     const uint32_t pixel = out / spp, sample = out - pixel * spp;
@@ -1423,27 +1451,41 @@ constexpr int kMaxDepth = 64;
 
 constexpr size_t kStaticLds = 256;  // the action table and alignment slack
 
+// Raises a kernel's dynamic-LDS limit to `lds` bytes on the current device
+// when it is above the default 64 KiB and above what was set before (once
+// per kernel, device and size, not on every launch).
+hipError_t AllowLds(const void* fn, size_t lds) {
+  if (lds <= 64 * 1024) return hipSuccess;
+  static std::mutex mu;
+  static std::map<std::pair<const void*, int>, size_t> allowed;
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  std::lock_guard<std::mutex> lock(mu);
+  size_t& have = allowed[{fn, dev}];
+  if (have >= lds) return hipSuccess;
+  e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
+  if (e == hipSuccess) have = lds;
+  return e;
+}
+
 template <bool kFast, bool kCount, bool kLds, bool kBatch>
 hipError_t Launch(const RenderParams& p, const LaunchShape& s, hipStream_t stream) {
   auto fn = kBatch ? BatchRenderKernel<kFast, kCount>
-                   : (s.block < 1024 ? RenderKernel<kFast, kCount, kLds, true> : RenderKernel<kFast, kCount, kLds, false>);
-  if (s.lds > 64 * 1024) {
-    const hipError_t e =
-        hipFuncSetAttribute(reinterpret_cast<const void*>(fn), hipFuncAttributeMaxDynamicSharedMemorySize,
-                            static_cast<int>(s.lds));
-    if (e != hipSuccess) return e;
-  }
+            : (!kLds && s.cacheOps > 0)
+                ? (s.block < 1024 ? RenderKernel<kFast, kCount, false, true, true>
+                                  : RenderKernel<kFast, kCount, false, false, true>)
+                : (s.block < 1024 ? RenderKernel<kFast, kCount, kLds, true> : RenderKernel<kFast, kCount, kLds, false>);
+  hipError_t e = AllowLds(reinterpret_cast<const void*>(fn), s.lds);
+  if (e != hipSuccess) return e;
   hipLaunchKernelGGL(fn, dim3(s.grid), dim3(s.block), s.lds, stream, p);
-  hipError_t e = hipGetLastError();
+  e = hipGetLastError();
   if (e != hipSuccess || p.budget <= 0) return e;
   // the heavy rays the first kernel handed over
   auto coop = CoopKernel<kFast, kCount, kBatch>;
   const size_t lds = CoopLaunchLds(p);
-  if (lds > 64 * 1024) {
-    e = hipFuncSetAttribute(reinterpret_cast<const void*>(coop), hipFuncAttributeMaxDynamicSharedMemorySize,
-                            static_cast<int>(lds));
-    if (e != hipSuccess) return e;
-  }
+  e = AllowLds(reinterpret_cast<const void*>(coop), lds);
+  if (e != hipSuccess) return e;
   hipLaunchKernelGGL(coop, dim3(p.coopWaves), dim3(64), lds, stream, p);
   return hipGetLastError();
 }
@@ -1451,6 +1493,7 @@ hipError_t Launch(const RenderParams& p, const LaunchShape& s, hipStream_t strea
 struct DeviceLimits {
   int cus = 0;
   size_t ldsPerBlock = 0;
+  size_t ldsPerCu = 0;
 };
 
 DeviceLimits Limits(int dev) {
@@ -1459,6 +1502,9 @@ DeviceLimits Limits(int dev) {
   if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess) l.cus = v;
   if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess)
     l.ldsPerBlock = static_cast<size_t>(v);
+  if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) == hipSuccess)
+    l.ldsPerCu = static_cast<size_t>(v);
+  if (l.ldsPerCu < l.ldsPerBlock) l.ldsPerCu = l.ldsPerBlock;
   return l;
 }
 
@@ -1493,11 +1539,12 @@ size_t CoopLdsBytes(int32_t ops, int32_t leaves, uint32_t* queueCap) {
   return bytes + kStaticLds <= limit ? bytes : 0;
 }
 
-hipError_t DeviceShapeLimits(int* computeUnits, size_t* ldsPerBlock) {
+hipError_t DeviceShapeLimits(int* computeUnits, size_t* ldsPerBlock, size_t* ldsPerCu) {
   const DeviceLimits* l = CurrentLimits();
   if (!l) return hipErrorNoDevice;
   *computeUnits = l->cus;
   *ldsPerBlock = l->ldsPerBlock;
+  if (ldsPerCu) *ldsPerCu = l->ldsPerCu;
   return hipSuccess;
 }
 
@@ -1515,7 +1562,28 @@ hipError_t ChooseShape(int32_t depthIn, size_t sceneBytes, const LaunchConfig& c
   LaunchShape s;
   bool lds = false;
   int block = cfg.block > 0 ? cfg.block : 0;
-  if (cfg.sceneInLds != 0 && sceneBytes > 0) {
+  if (cfg.sceneInLds == 2 && sceneBytes > 0 && ops > 0) {
+    // Scene in global memory, the top of the tree cached in LDS (operators
+    // are numbered in level order, so the cache is a prefix): 1024 / block
+    // workgroups a CU (four waves a SIMD) share the CU's LDS, each holding
+    // its lanes' stacks and as many operator records as the rest takes.  (A
+    // scene without operators has nothing to cache: a plain global launch.)
+    const int b = block ? block : 1024;
+    const size_t perCu = static_cast<size_t>(std::max(1, 1024 / b));
+    const size_t ldsCu = lim.ldsPerCu > 0 ? lim.ldsPerCu : ldsCap;
+    const size_t share = std::min(ldsCu / perCu, ldsCap);
+    const size_t stack = depth * static_cast<size_t>(b) * 8;
+    if (stack + kStaticLds + 64 > share) return hipErrorInvalidValue;
+    const size_t fit = (share - stack - kStaticLds) / 64;
+    s.block = b;
+    s.sceneInLds = false;
+    s.cacheOps = static_cast<int32_t>(std::min<size_t>(fit, static_cast<size_t>(ops)));
+    s.lds = stack + static_cast<size_t>(s.cacheOps) * 64;
+    s.grid = cus * static_cast<int>(perCu);
+    *shape = s;
+    return hipSuccess;
+  }
+  if (cfg.sceneInLds != 0 && cfg.sceneInLds != 2 && sceneBytes > 0) {
     // The largest workgroup that fits: most waves sharing one scene copy.
     // 1024 lanes (4 waves per SIMD) take each tile about twice as long as
     // 512, which the heavy-ray hand-over hides on scenes with long rays but

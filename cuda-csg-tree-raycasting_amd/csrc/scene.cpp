@@ -236,10 +236,24 @@ FlatScene Flatten(const CSGTree& tree) {
     std::memcpy(&v, &u, 4);
     return v;
   };
-  // operator index of every operator node, in preorder
+  // Operator index of every operator node, in level order from the root:
+  // any prefix [0, n) of the operators is then the top of the tree, which a
+  // launch can keep in LDS when the whole scene does not fit.  (The
+  // traversal follows explicit child and parent links, so the numbering does
+  // not change a result.)
   std::vector<int32_t> opIndex(tree.nodes.size(), -1);
-  for (int32_t i = 0; i < f.nNodes; i++)
-    if (tree.nodes[i].primitiveIdx == -1) opIndex[i] = f.nOps++;
+  if (f.nNodes > 0 && tree.nodes[0].primitiveIdx == -1) {
+    std::vector<int32_t> level{0};
+    while (!level.empty()) {
+      std::vector<int32_t> next;
+      for (int32_t i : level) {
+        opIndex[i] = f.nOps++;
+        for (int32_t c : {tree.nodes[i].left, tree.nodes[i].right})
+          if (tree.nodes[c].primitiveIdx == -1) next.push_back(c);
+      }
+      level.swap(next);
+    }
+  }
   auto desc = [&](int32_t node) {
     const CSGNode& n = tree.nodes[node];
     const uint32_t index = n.primitiveIdx != -1 ? static_cast<uint32_t>(n.primitiveIdx)

@@ -67,7 +67,8 @@ struct CSGTree {
 // Device layout.  Traversal only ever stands on operators: a step down reads
 // the operator's record, which holds both children's boxes and descriptors;
 // a step up reads the parent's descriptor word.  Operators are numbered in
-// preorder (0 = the root when the root is an operator).
+// level order from the root (0 = the root when the root is an operator), so
+// that every prefix of the operator records is the top of the tree.
 //
 //   ops[4k+0] = (L.lo.x, L.lo.y, L.lo.z, R.lo.x)
 //   ops[4k+1] = (L.hi.x, L.hi.y, L.hi.z, R.lo.y)

@@ -107,7 +107,10 @@ typedef enum csg_mode { CSG_MODE_EXACT = 0, CSG_MODE_FAST = 1 } csg_mode;
 typedef struct csg_render_opts {
   int32_t mode;         /* csg_mode */
   int32_t block;        /* threads per workgroup: 0 = auto, else 256, 512 or 1024 */
-  int32_t scene_placement; /* 0 = auto, 1 = read the scene from global memory, 2 = stage it in LDS */
+  int32_t scene_placement; /* 0 = auto, 1 = read the scene from global memory, 2 = stage it in LDS,
+                              3 = global memory with the top of the tree (as many operator records,
+                              in level order, as fit beside the stacks) staged in LDS: for scenes
+                              too large for LDS; single-scene renders only */
   int32_t counters;     /* 1: also write per-ray event counters (testing) */
   int32_t* counter_out; /* device, 6 int32 per pixel: GoTo, Compute, box, sphere, cylinder, cube */
   const uint32_t* tile_order; /* device, optional: order in which the ceil(w/8) x ceil(h/8) tiles of
@@ -234,13 +237,18 @@ csg_status csg_batch_create(const csg_job* jobs, int32_t njobs, const csg_render
 csg_status csg_batch_update(csg_batch* batch, const csg_job* jobs, int32_t njobs, void* stream);
 csg_status csg_batch_launch(csg_batch* batch, void* stream);
 /* Times candidate launch shapes on the batch (workgroup 256/512/1024 lanes x
- * row-major/Morton tile walk x given/heaviest-first job order, `reps` launches
- * each), keeps the fastest for later launches and reports it. */
+ * row-major/Morton tile walk x given/heaviest-first job order x LDS stack
+ * depth: one launch whose stacks fit the deepest tree, or two depth classes
+ * split at the depth that halves the batch's work (pixels x depth; failing
+ * that, below the deepest trees), each launch's stacks sized to its own
+ * deepest tree; `reps` launches each), keeps the fastest for later launches
+ * and reports it. */
 typedef struct csg_batch_tuning {
   int32_t block;
   int32_t tile_walk;
   int32_t heaviest_first;
   float ms;  /* mean time of one launch of the batch with this choice */
+  int32_t depth_split; /* 0: one launch; d > 0: trees of <= d levels launch apart from deeper ones */
 } csg_batch_tuning;
 csg_status csg_batch_autotune(csg_batch* batch, int32_t reps, void* stream, csg_batch_tuning* chosen);
 csg_status csg_batch_free(csg_batch* batch);
@@ -263,6 +271,7 @@ typedef struct csg_launch_info {
   int32_t coop_grace;    /* tail grace (percent) a 1920x1080 frame would get (the default is
                             950 x waves / tiles percent) */
   int64_t coop_lds_bytes; /* its dynamic LDS per workgroup */
+  int32_t cache_ops;      /* operator records cached in LDS (scene_placement 3), else 0 */
 } csg_launch_info;
 csg_status csg_render_plan(const csg_scene* scene, const csg_render_opts* opts, csg_launch_info* info);
 

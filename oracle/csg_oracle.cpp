@@ -428,7 +428,45 @@ static inline bool near_abs_nz(double a, double b, double eps) {  // |a-b| <= ep
   return std::isfinite(a) && a != b && std::fabs(a - b) <= eps;
 }
 
-constexpr double kEpsTie = 1e-4, kEpsBox = 1e-5, kEpsRoot = 1e-5;
+constexpr double kEpsRoot = 1e-5;
+// Combine ties and box / slab limits: relative margins of the flagger.
+static double kEpsTie = 1e-4, kEpsBox = 1e-5;
+
+// Near-tangent test of the flagger.  g_tangentEta = 0: the discriminant
+// within kEpsRoot of b^2 + |c|.  g_tangentEta > 0: the ray's closest
+// approach p to the sphere's centre (the cylinder's axis) within
+// eta * max(distance, r) of the radius r, i.e. a sideways move of the ray by
+// eta times its length to the primitive would carry it across the surface's
+// silhouette; cylinder rays within the reference's 1e-5 clamp of A keep the
+// discriminant test (the clamped quadratic is no geometric tangency).
+static double g_tangentEta = 0;
+
+template <class Real>
+static bool sphere_near_tangent(const Ray<Real>& ray, double x, double y, double z, double r, double disc, double b,
+                                double cc) {
+  if (g_tangentEta <= 0) return std::fabs(disc) <= kEpsRoot * (b * b + std::fabs(cc));
+  const double ox = (double)ray.origin.x - x, oy = (double)ray.origin.y - y, oz = (double)ray.origin.z - z;
+  const double dx = ray.direction.x, dy = ray.direction.y, dz = ray.direction.z;
+  const double bd = ox * dx + oy * dy + oz * dz;
+  const double fx = ox - bd * dx, fy = oy - bd * dy, fz = oz - bd * dz;
+  const double p = std::sqrt(fx * fx + fy * fy + fz * fz), dist = std::sqrt(ox * ox + oy * oy + oz * oz);
+  return std::fabs(p - r) <= g_tangentEta * std::max(dist, r);
+}
+
+template <class Real>
+static bool cylinder_near_tangent(const Ray<Real>& ray, V3<Real> C, V3<Real> V, double r, double disc, double b,
+                                  double acc, double a_true) {
+  if (g_tangentEta <= 0 || a_true < 0.00001) return std::fabs(disc) <= kEpsRoot * (b * b + std::fabs(acc));
+  const double ox = (double)ray.origin.x - C.x, oy = (double)ray.origin.y - C.y, oz = (double)ray.origin.z - C.z;
+  const double dx = ray.direction.x, dy = ray.direction.y, dz = ray.direction.z;
+  const double vx = V.x, vy = V.y, vz = V.z;
+  const double ex = dy * vz - dz * vy, ey = dz * vx - dx * vz, ez = dx * vy - dy * vx;
+  const double gx = oy * vz - oz * vy, gy = oz * vx - ox * vz, gz = ox * vy - oy * vx;
+  const double A = ex * ex + ey * ey + ez * ez, tc = -(ex * gx + ey * gy + ez * gz) / A;
+  const double fx = gx + tc * ex, fy = gy + tc * ey, fz = gz + tc * ez;
+  const double p = std::sqrt(fx * fx + fy * fy + fz * fz), dist = std::sqrt(ox * ox + oy * oy + oz * oz);
+  return std::fabs(p - r) <= g_tangentEta * std::max(dist, r);
+}
 
 // C++ truncating float->int conversion with the saturation AMD's v_cvt_i32_f32
 // applies outside the int range (NaN -> 0); inside the range it is (int)q.
@@ -486,7 +524,7 @@ template <class Real> static bool sphere_hit(Ctx<Real>& c, const PrimR<Real>& sp
   Real b = dot(oc, ray.direction);
   Real cc = dot(oc, oc) - sphere.radius * sphere.radius;
   Real discriminant = b * b - cc;
-  if (c.flag && std::fabs((double)discriminant) <= kEpsRoot * ((double)b * b + std::fabs((double)cc)))
+  if (c.flag && sphere_near_tangent(ray, sphere.x, sphere.y, sphere.z, sphere.radius, discriminant, b, cc))
     c.degen |= DG_ROOT;
   if (discriminant < 0) return false;
   Real temp = (-b - std::sqrt(discriminant));
@@ -525,7 +563,8 @@ template <class Real> static bool cylinder_hit(Ctx<Real>& c, const PrimR<Real>& 
   Real b = (dot(ray.direction, OC) - dot(ray.direction, V) * dot(OC, V));
   Real discriminant = b * b - a * cc;
   if (c.flag) {
-    if (std::fabs((double)discriminant) <= kEpsRoot * ((double)b * b + std::fabs((double)a * cc))) c.degen |= DG_ROOT;
+    if (cylinder_near_tangent(ray, C, V, radius, discriminant, b, (double)a * cc, 1 - (double)aHelp * aHelp))
+      c.degen |= DG_ROOT;
     if (near_rel(1 - (double)aHelp * aHelp, 0.00001f, kEpsBox)) c.degen |= DG_ROOT;
   }
   if (discriminant < 0) return false;
@@ -1116,6 +1155,13 @@ using namespace oracle;
 extern "C" {
 
 void oracle_set_mutation(int m) { g_mutation = m; }
+
+void oracle_set_tangent_eta(double eta) { g_tangentEta = eta; }
+
+void oracle_set_flag_margins(double tie, double box) {
+  kEpsTie = tie;
+  kEpsBox = box;
+}
 
 int oracle_parse(const char* text, size_t len, void** out, char* err, size_t errlen) {
   try {

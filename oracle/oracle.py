@@ -50,6 +50,8 @@ def lib():
         L.oracle_classify.restype = ctypes.c_int
         L.oracle_classify.argtypes = [_P, _F, ctypes.c_int, ctypes.c_int, ctypes.c_double, _P, _P, _P, _P]
         L.oracle_set_mutation.argtypes = [ctypes.c_int]
+        L.oracle_set_tangent_eta.argtypes = [ctypes.c_double]
+        L.oracle_set_flag_margins.argtypes = [ctypes.c_double, ctypes.c_double]
         L.oracle_render_samples.restype = ctypes.c_int
         L.oracle_render_samples.argtypes = [_P, _F, _F, _F, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint32,
                                             ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
@@ -132,7 +134,7 @@ class OracleScene:
         return out
 
     def render_samples(self, cam6, light3, phong8, width, height, spp, seed, bits=32, threads=0, rows=None,
-                       counters=False):
+                       counters=False, degen=False):
         """spp jittered rays per pixel; arrays shaped (H, W*spp, ...): sample s of pixel x at column x*spp+s."""
         if threads <= 0:
             threads = min(os.cpu_count() or 1, 16)
@@ -146,10 +148,12 @@ class OracleScene:
         }
         if counters:
             out["counters"] = np.zeros((height, w, 6), np.int32)
+        if degen:
+            out["degen"] = np.zeros((height, w), np.uint8)
         st = lib().oracle_render_samples(
             self.h, _f32(cam6), _f32(light3), _f32(phong8), width, height, spp, seed, bits, threads, r0, r1, rs,
             out["rgba"].ctypes.data, out["depth"].ctypes.data, out["prim_id"].ctypes.data, out["flags"].ctypes.data,
-            out["counters"].ctypes.data if counters else None, None)
+            out["counters"].ctypes.data if counters else None, out["degen"].ctypes.data if degen else None)
         if st != 0:
             raise ValueError("oracle_render_samples: bad arguments")
         return out
@@ -167,6 +171,17 @@ class OracleScene:
 
 def jitter(seed, pixel, sample, axis):
     return float(lib().oracle_jitter(seed, pixel, sample, axis))
+
+
+def set_tangent_eta(eta: float):
+    """Near-tangent test of the flagger: 0 = discriminant relative to b^2;
+    eta > 0 = closest approach within eta * distance of the radius."""
+    lib().oracle_set_tangent_eta(float(eta))
+
+
+def set_flag_margins(tie: float, box: float):
+    """Relative margins of the flagger's combine-tie and box/slab tests."""
+    lib().oracle_set_flag_margins(float(tie), float(box))
 
 
 def set_mutation(m: int):

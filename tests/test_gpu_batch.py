@@ -253,3 +253,47 @@ def test_autotune_keeps_results(gpu):
     batch.launch()
     torch.cuda.synchronize()
     _check(specs, ref)
+
+
+def test_autotune_depth_classes(gpu):
+    """A batch of shallow random trees and deep operator chains (32 levels):
+    the autotuner also times the batch split into two depth classes, each
+    launch's LDS stacks sized to its own deepest tree.  The split
+    configuration is timed last, so the outputs right after tuning come from
+    it; they, and the outputs of the chosen configuration, equal the solo
+    renders, and in exact mode the oracle."""
+    from deep_trees import left_deep_difference, right_deep_union
+    print("\n[case] autotune with depth classes (shallow trees + 32-level chains)", flush=True)
+    light, ph = C.light(S.LIGHT_TOWARD), C.phong(*S.PHONG)
+    texts = [t for t, _, _ in S.batch_mix(10, seed=31)]
+    texts += [left_deep_difference(32, seed=s) for s in range(3)] + [right_deep_union(32, seed=s) for s in range(3)]
+    rng = np.random.default_rng(32)
+    specs = []
+    for t in texts:
+        cam = C.camera_look_at(S.orbit_eye(rng, 4.5, 70.0), (0, 0, 0), 60)
+        w, h = int(rng.integers(48, 160)), int(rng.integers(48, 120))
+        specs.append((C.Scene(t), cam, light, ph, w, h, C.alloc_outputs(w, h)))
+    depths = sorted(sc.info().op_depth for sc, *_ in specs)
+    assert depths[0] <= 10 and depths[-1] == 32
+    ref = [_np(sc.render(cam, l, p, w, h, mode=C.MODE_EXACT)) for sc, cam, l, p, w, h, _ in specs]
+    torch.cuda.synchronize()
+    batch = C.Batch(C.make_jobs(specs), mode=C.MODE_EXACT)
+    t = batch.autotune(reps=1)
+    torch.cuda.synchronize()
+    print(f"chosen block={t.block} walk={t.tile_walk} heaviest_first={t.heaviest_first} "
+          f"depth_split={t.depth_split} ms={t.ms:.3f}")
+    assert t.depth_split in (0,) + tuple(d for d in depths if d < 32)
+    _check(specs, ref)  # the last timed configuration: two depth classes
+    for spec in specs:
+        for x in spec[6].values():
+            x.fill_(0)
+    batch.launch()
+    torch.cuda.synchronize()
+    _check(specs, ref)
+    batch.close()
+    for (sc, cam, l, p, w, h, outs), text in zip(specs, texts):
+        g = _np(outs)
+        o = O.OracleScene(text).render(C.cam_tuple(cam), S.LIGHT_TOWARD, O.phong8(*S.PHONG), w, h, bits=32,
+                                       stack_cap=2 * sc.info().op_depth + 1)
+        assert np.array_equal(g["prim_id"], o["prim_id"])
+        assert np.array_equal(g["depth"].view(np.uint32), o["depth"].view(np.uint32))

@@ -35,7 +35,7 @@ def _label(name, setup):
     print(f"\n[case] {name} setup={setup}", flush=True)
 
 
-@pytest.mark.parametrize("placement", [C.PLACE_GLOBAL, C.PLACE_LDS])
+@pytest.mark.parametrize("placement", [C.PLACE_GLOBAL, C.PLACE_LDS, C.PLACE_CACHE])
 @pytest.mark.parametrize("setup", [0, 1])
 @pytest.mark.parametrize("name", scene_names())
 def test_exact_matches_fp32_oracle_bitwise(gpu, name, setup, placement):
@@ -247,6 +247,13 @@ def test_launch_plan(gpu):
     assert g.scene_in_lds == 0
     lds = s.plan(placement=C.PLACE_LDS)
     assert lds.scene_in_lds == 1 and lds.lds_bytes <= lds.lds_limit
+    # the top of the tree cached in LDS: as many of its 512 operator records as fit beside the stacks
+    for block in (1024, 512, 256):
+        c = s.plan(placement=C.PLACE_CACHE, block=block)
+        assert c.scene_in_lds == 0 and c.block == block and c.lds_bytes <= c.lds_limit
+        assert 0 < c.cache_ops <= 512 and c.lds_bytes == 10 * block * 8 + 64 * c.cache_ops
+        if block == 1024:  # one workgroup a CU: all the LDS beside the stacks
+            assert c.cache_ops == min(512, (c.lds_limit - 10 * block * 8 - 256) // 64)
     # the largest workgroup whose stacks and scene copy fit, and the default tail grace of a 1920x1080
     # frame (240 x 135 tiles): 950 % x waves / tiles
     assert auto.scene_in_lds == 1 and auto.block == 1024 and auto.coop_grace == 120

@@ -27,16 +27,20 @@ def stress():
     return text, C.Scene(text)
 
 
+@pytest.mark.parametrize("placement", [C.PLACE_AUTO, C.PLACE_CACHE], ids=["auto", "lds-cache"])
 @pytest.mark.parametrize("coop,when", [(-1, 0), (0, 0), (256, 1)])
-def test_stress_samples_match_fp32_oracle(gpu, stress, coop, when):
-    print(f"\n[case] stress 8191 nodes 160x90 2 spp exact coop={coop} when={when}", flush=True)
+def test_stress_samples_match_fp32_oracle(gpu, stress, coop, when, placement):
+    print(f"\n[case] stress 8191 nodes 160x90 2 spp exact coop={coop} when={when} placement={placement}", flush=True)
     text, scene = stress
     cam = C.camera_look_at(S.VIEW_EYE, S.VIEW_TARGET, S.VIEW_VFOV_DEG)
     w, h, spp, seed = 160, 90, 2, 4095
-    plan = scene.plan(mode=C.MODE_EXACT, coop=coop)
+    plan = scene.plan(mode=C.MODE_EXACT, coop=coop, placement=placement)
     assert not plan.scene_in_lds
+    if placement == C.PLACE_CACHE:
+        # part of the tree in LDS (level order: the top levels), the rest from global memory
+        assert 0 < plan.cache_ops < scene.info().nodes // 2 and plan.lds_bytes <= plan.lds_limit
     g = _np(scene.render(cam, C.light(S.LIGHT_TOWARD), C.phong(*S.PHONG), w, h, mode=C.MODE_EXACT, counters=True,
-                         samples=spp, seed=seed, coop=coop, coop_when=when))
+                         samples=spp, seed=seed, coop=coop, coop_when=when, placement=placement))
     o = O.OracleScene(text).render_samples(C.cam_tuple(cam), S.LIGHT_TOWARD, O.phong8(*S.PHONG), w, h, spp, seed,
                                            bits=32, counters=True)
     assert np.array_equal(g["prim_id"], o["prim_id"])
