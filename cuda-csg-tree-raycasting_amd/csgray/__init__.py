@@ -128,6 +128,9 @@ _SIGNATURES = [
     ("csg_render_ex", ctypes.c_int, [_P, ctypes.POINTER(Camera), ctypes.POINTER(Light), ctypes.POINTER(Phong),
                                      ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(Outputs),
                                      ctypes.POINTER(RenderOpts), _P]),
+    ("csg_render_noalloc", ctypes.c_int, [_P, ctypes.POINTER(Camera), ctypes.POINTER(Light), ctypes.POINTER(Phong),
+                                          ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(Outputs),
+                                          ctypes.POINTER(RenderOpts), _P]),
     ("csg_sync", ctypes.c_int, [_P]),
     ("csg_coop_rays", ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int64)]),
     ("csg_camera_move", ctypes.c_int, [ctypes.POINTER(Camera), ctypes.c_float, ctypes.c_float, ctypes.c_float]),

@@ -190,6 +190,18 @@ csg_status csg_render(const csg_scene* scene, const csg_camera* cam, const csg_l
 csg_status csg_render_ex(const csg_scene* scene, const csg_camera* cam, const csg_light* light,
                          const csg_phong* phong, int32_t width, int32_t height, const csg_outputs* out,
                          const csg_render_opts* opts, void* stream);
+/* csg_render_ex that allocates nothing: no counterpart in the reference;
+ * for callers that cannot allocate device memory (code under stream capture,
+ * tight serving loops).  Same arguments and results as csg_render_ex.  Its
+ * scratch (work-queue words, a heavy-ray list of 4,194,304 entries and a
+ * 16 MiB resume area, about 48 MiB) is one static device array per GPU,
+ * held until the process ends; calls of more than 4,194,304 rays
+ * (width x height x samples) fail with CSG_E_ARG.  Every call on one GPU
+ * shares that array, so calls that overlap on one GPU (several streams or
+ * host threads) race: serialise them. */
+csg_status csg_render_noalloc(const csg_scene* scene, const csg_camera* cam, const csg_light* light,
+                              const csg_phong* phong, int32_t width, int32_t height, const csg_outputs* out,
+                              const csg_render_opts* opts, void* stream);
 csg_status csg_sync(void* stream);
 /* Each (device, stream) pair gets a small workspace (tile queue, heavy-ray
  * list) on its first render, cleared with hipMemsetAsync on that stream; the
