@@ -38,10 +38,13 @@
 //
 // Besides the restatement this file holds two checkers:
 //   * a degenerate-decision flagger (FP64 runs): marks rays whose result hangs
-//     on a near-tie — two hit distances within a relative 1e-4 at a combine
-//     decision, a grazing discriminant, a box or slab limit within a relative
-//     1e-5 of another limit, a cylinder axial coordinate at an end cap, a cube
-//     normal component at a truncation boundary, or a tangent classification;
+//     on a near-tie — two hit distances within a relative 1e-5 at a combine
+//     decision, a ray within 2e-6 of its length of a sphere's or cylinder's
+//     silhouette, a box or slab limit within a relative 1e-6 of another
+//     limit, a cylinder axial coordinate at an end cap, a cube normal
+//     component at a truncation boundary, or a tangent classification (the
+//     margins can be changed for measurements: oracle_set_tangent_eta,
+//     oracle_set_flag_margins);
 //   * an interval classifier (FP64) that evaluates the CSG expression on the
 //     sorted boundary points of every primitive along the ray, independent of
 //     the Kensler state machine, for tiny images.
@@ -89,9 +92,9 @@ enum HitFlags { Enter = 2, Exit = 4, Miss = 8, Flip = 16, Flag1 = 32, Flag2 = 64
 
 // Degenerate-decision flag bits (per ray).
 enum DegenBits {
-  DG_TIE = 1,        // combine decision between two hits closer than 1e-4 relative
-  DG_BOX = 2,        // box / cube slab limits within 1e-5 relative of a decision limit
-  DG_ROOT = 4,       // discriminant near zero, or a root within 1e-5 of tmin
+  DG_TIE = 1,        // combine decision between two hits closer than kEpsTie relative
+  DG_BOX = 2,        // box / cube slab limits within kEpsBox relative of a decision limit
+  DG_ROOT = 4,       // ray at a sphere's or cylinder's silhouette (near_tangent), or a root near tmin
   DG_CYL_END = 8,    // cylinder axial coordinate at 0 or h, or cap denominator at 1e-4
   DG_CUBE_N = 16,    // cube normal component at the integer-truncation boundary
   DG_TANGENT = 32,   // |normal . d| tiny at the enter/exit classification
@@ -428,18 +431,30 @@ static inline bool near_abs_nz(double a, double b, double eps) {  // |a-b| <= ep
   return std::isfinite(a) && a != b && std::fabs(a - b) <= eps;
 }
 
-constexpr double kEpsRoot = 1e-5;
-// Combine ties and box / slab limits: relative margins of the flagger.
-static double kEpsTie = 1e-4, kEpsBox = 1e-5;
+// Relative margins of the flagger: a root within kEpsRoot of tmin; the
+// cylinder's clamp of A, axial coordinate at a cap and cap denominator within
+// kEpsCyl; a combine decision between two hits within kEpsTie; a box or cube
+// slab limit within kEpsBox of another limit.  Fast mode's depth error stays
+// near 1e-6 and its slab limits within a few ulps, and on the benchmark
+// frames (cheese, the 256-tree mix, the stress scene, Wikipedia) it agrees
+// with the FP64 oracle on every ray these margins leave unflagged also at
+// half of kEpsTie / kEpsBox and g_tangentEta.  (The margins of an earlier
+// design, 1e-4 for ties, 1e-5 for slabs and a discriminant within 1e-5 of
+// b^2 + |c|, flagged 11.8% of the stress scene's rays, 2.6% of the cheese
+// frame's.)
+constexpr double kEpsRoot = 1e-5, kEpsCyl = 1e-5;
+static double kEpsTie = 1e-5, kEpsBox = 1e-6;
 
-// Near-tangent test of the flagger.  g_tangentEta = 0: the discriminant
-// within kEpsRoot of b^2 + |c|.  g_tangentEta > 0: the ray's closest
-// approach p to the sphere's centre (the cylinder's axis) within
-// eta * max(distance, r) of the radius r, i.e. a sideways move of the ray by
-// eta times its length to the primitive would carry it across the surface's
-// silhouette; cylinder rays within the reference's 1e-5 clamp of A keep the
-// discriminant test (the clamped quadratic is no geometric tangency).
-static double g_tangentEta = 0;
+// Near-tangent test of the flagger: the ray's closest approach p to the
+// sphere's centre (the cylinder's axis) within g_tangentEta * max(distance,
+// r) of the radius r, i.e. a sideways move of the ray by 2e-6 of its length
+// to the primitive (about 30 FP32 ulps) would carry it across the surface's
+// silhouette.  Cylinder rays within the reference's 1e-5 clamp of A keep a
+// discriminant test (the clamped quadratic is no geometric tangency), as does
+// g_tangentEta = 0: the discriminant within kEpsRoot of b^2 + |c| (a
+// criterion relative to the distance, not to the primitive, which marks
+// every hit on a primitive whose radius is small against its distance).
+static double g_tangentEta = 2e-6;
 
 template <class Real>
 static bool sphere_near_tangent(const Ray<Real>& ray, double x, double y, double z, double r, double disc, double b,
@@ -565,7 +580,7 @@ template <class Real> static bool cylinder_hit(Ctx<Real>& c, const PrimR<Real>& 
   if (c.flag) {
     if (cylinder_near_tangent(ray, C, V, radius, discriminant, b, (double)a * cc, 1 - (double)aHelp * aHelp))
       c.degen |= DG_ROOT;
-    if (near_rel(1 - (double)aHelp * aHelp, 0.00001f, kEpsBox)) c.degen |= DG_ROOT;
+    if (near_rel(1 - (double)aHelp * aHelp, 0.00001f, kEpsCyl)) c.degen |= DG_ROOT;
   }
   if (discriminant < 0) return false;
   Real t1 = (-b - std::sqrt(discriminant)) / (a), t2 = (-b + std::sqrt(discriminant)) / (a);
@@ -573,7 +588,7 @@ template <class Real> static bool cylinder_hit(Ctx<Real>& c, const PrimR<Real>& 
   if (c.flag) {
     double hs = std::max((double)std::fabs(h), 1e-3);
     for (double m : {(double)m1, (double)m2})
-      if (std::fabs(m) <= kEpsBox * hs || std::fabs(m - h) <= kEpsBox * hs) c.degen |= DG_CYL_END;
+      if (std::fabs(m) <= kEpsCyl * hs || std::fabs(m - h) <= kEpsCyl * hs) c.degen |= DG_CYL_END;
   }
   if ((m1 < 0 && m2 < 0) || (m1 > h && m2 > h)) return false;
   Real temp = t1;
@@ -584,7 +599,7 @@ template <class Real> static bool cylinder_hit(Ctx<Real>& c, const PrimR<Real>& 
   auto caps = [&]() {
     if (m < 0) {
       Real den = dot(ray.direction, g_mutation == 8 ? V : -V);
-      if (c.flag && near_rel(std::fabs((double)den), 0.0001f, kEpsBox)) c.degen |= DG_CYL_END;
+      if (c.flag && near_rel(std::fabs((double)den), 0.0001f, kEpsCyl)) c.degen |= DG_CYL_END;
       if (std::fabs(den) < Real(0.0001f)) {
         skip = true;
       } else {
@@ -595,7 +610,7 @@ template <class Real> static bool cylinder_hit(Ctx<Real>& c, const PrimR<Real>& 
     }
     if (m > h) {
       Real den = dot(ray.direction, V);
-      if (c.flag && near_rel(std::fabs((double)den), 0.0001f, kEpsBox)) c.degen |= DG_CYL_END;
+      if (c.flag && near_rel(std::fabs((double)den), 0.0001f, kEpsCyl)) c.degen |= DG_CYL_END;
       if (std::fabs(den) < Real(0.0001f)) {
         skip = true;
       } else {

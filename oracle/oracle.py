@@ -18,7 +18,8 @@ SAN_LIB_PATH = os.path.join(ORACLE_DIR, "_build", "libcsg_oracle_san.so")
 
 # per-ray counter columns
 COUNTER_NAMES = ("goto", "compute", "box", "sphere", "cylinder", "cube")
-# degenerate-decision flag bits
+# degenerate-decision flag bits (combine tie, box/slab limit, silhouette or root at tmin, cylinder end,
+# cube-normal truncation, tangent classification, stack overflow)
 DG_TIE, DG_BOX, DG_ROOT, DG_CYL_END, DG_CUBE_N, DG_TANGENT, DG_OVERFLOW = 1, 2, 4, 8, 16, 32, 64
 
 _LIB = None
@@ -174,13 +175,15 @@ def jitter(seed, pixel, sample, axis):
 
 
 def set_tangent_eta(eta: float):
-    """Near-tangent test of the flagger: 0 = discriminant relative to b^2;
-    eta > 0 = closest approach within eta * distance of the radius."""
+    """Near-tangent test of the flagger (for measurements; default 2e-6): the
+    ray's closest approach within eta * distance of the radius; 0 = the
+    discriminant within 1e-5 of b^2 + |c|."""
     lib().oracle_set_tangent_eta(float(eta))
 
 
 def set_flag_margins(tie: float, box: float):
-    """Relative margins of the flagger's combine-tie and box/slab tests."""
+    """Relative margins of the flagger's combine-tie and box/slab tests (for
+    measurements; defaults 1e-5 and 1e-6)."""
     lib().oracle_set_flag_margins(float(tie), float(box))
 
 

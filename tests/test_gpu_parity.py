@@ -190,9 +190,10 @@ def test_fast_vs_fp64_stress_4k(gpu):
     fast mode against the FP64 oracle on every 16th row (518 k rays): hit ids
     on unflagged rays and depth within 1e-4.  The reference's own FP32
     arithmetic (the FP32 oracle, which exact mode equals bit for bit) is only
-    reported: it exceeds 1e-4 on a few unflagged near-tangent side exits of
-    thin cylinders seen almost along their axis (3.05e-4), where its
-    |OC|^2 - (OC.V)^2 cancels; fast mode's closest-approach form does not."""
+    reported: its |OC|^2 - (OC.V)^2 cancels on thin cylinders, which moves
+    depths by up to 3e-4 (near-tangent side exits seen almost along the axis)
+    and flips about 0.07% of the sample's hits on rays that are not near a
+    tangent; fast mode's closest-approach form does neither."""
     from csgray import scenes as S
     _label("stress 3840x2160", 1)
     sname, cam, light, phong, cam6, l3, ph8 = setups()[1]
@@ -206,8 +207,9 @@ def test_fast_vs_fp64_stress_4k(gpu):
     accuracy_report("stress 3840x2160 reference FP32 arithmetic (every 16th row)", o32, o64, rows=rows)
     rate, drel, flagged = compare_fp64(g, o64, rows=rows)
     rate32, drel32, _ = compare_fp64(o32, o64, rows=rows)
-    print(f"agree(unflagged)={rate:.6f} depth_rel_max={drel:.2e} (reference FP32 {drel32:.2e}) flagged={flagged:.4%}")
-    assert rate >= AGREE_MIN and rate32 >= AGREE_MIN
+    print(f"agree(unflagged)={rate:.6f} depth_rel_max={drel:.2e} flagged={flagged:.4%} "
+          f"(the reference's FP32 arithmetic: agree(unflagged)={rate32:.6f} depth_rel_max={drel32:.2e})")
+    assert rate >= AGREE_MIN
     assert drel <= DEPTH_REL_TOL
 
 

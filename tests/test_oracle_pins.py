@@ -1,9 +1,10 @@
 """Pins for the oracle, independent of the restatement's own formulas.
 
 * Known answers measured on the unmodified reference sources (a host build of
-  RaycastingKernels.cu / CSGTree.cu / Camera.cpp, FP32, made once outside
-  this repository, whose figures are written into the tests below): hit
-  counts and mean per-ray event counts at 640x360.
+  RaycastingKernels.cu / CSGTree.cu / Camera.cpp behind a small header shim,
+  FP32, made once outside this repository, whose figures are written into
+  the tests below): hit counts and mean per-ray event counts at 640x360.
+  They corroborate; the checks below carry the weight.
 * Closed forms: cylinder axis = Ry(y) Rx(x) Rz(z) (0,1,0); leaf boxes; camera
   basis; the centre ray's analytic depth on single primitives.
 * An interval classifier that evaluates the CSG expression between the
@@ -12,7 +13,11 @@
   it on every ray it does not flag as degenerate.
 * Invariants: a far-away operand leaves the image unchanged.
 * Planted mistakes: each of eight plausible errors in the restatement fails
-  at least one of the pins above.
+  at least one of the pins above (the shading half has its own pins and
+  planted mistakes in test_oracle_shading.py).
+* The degenerate-decision flagger: every ray whose FP64 result a move of the
+  view by 1e-6 of its distance flips is flagged, and a tangency margin 1000
+  times too small is caught.
 """
 import math
 
@@ -260,3 +265,57 @@ def test_each_planted_mistake_fails_a_pin(mutation):
 def test_pins_pass_without_mistakes():
     O.set_mutation(0)
     assert all(p() for p in PINS)
+
+
+# ------------------------------------------------------------- the flagger ---
+
+def _view_moves(cam6, rel=1e-6):
+    """The camera moved by `rel` of its distance along each axis, and turned
+    by `rel` radians in pitch and in yaw (each way)."""
+    d = math.sqrt(sum(c * c for c in cam6[:3]))
+    out = []
+    for k in range(5):
+        for sgn in (1, -1):
+            v = list(cam6)
+            v[k] = float(np.float32(v[k] + sgn * rel * (d if k < 3 else 1.0)))
+            out.append(tuple(v))
+    return out
+
+
+def unflagged_flips(text, w, h, rel=1e-6):
+    """Rays whose FP64 hit id or enter/exit changes when the view moves by
+    `rel` (half the flagger's tangency margin), and how many of them the
+    unmoved render leaves unflagged."""
+    cam6 = view_cam6()
+    o = O.OracleScene(text)
+    base = o.render(cam6, LIGHT, PH8, w, h, bits=64, degen=True)
+    flipped = np.zeros((h, w), bool)
+    for v in _view_moves(cam6, rel):
+        r = o.render(v, LIGHT, PH8, w, h, bits=64)
+        flipped |= (r["prim_id"] != base["prim_id"]) | ((r["flags"] & 14) != (base["flags"] & 14))
+    return int(flipped.sum()), int((flipped & (base["degen"] == 0)).sum())
+
+
+def test_flags_cover_every_ray_a_small_move_of_the_view_flips():
+    """The degenerate set is meant to hold the rays whose result is not fixed
+    at FP32 input precision.  Moving the camera by 1e-6 of its distance (or
+    turning it by 1e-6 rad) flips the coplanar ties of the Wikipedia view and
+    the silhouette rays of the stress scene's small primitives; every flipped
+    ray must be flagged in the unmoved render."""
+    from csgray import scenes as S
+    for text, w, h, least in ((scene_text("testWikipedia.txt"), 320, 180, 100),
+                              (S.stress(4095, seed=4095), 640, 360, 1)):
+        flips, missed = unflagged_flips(text, w, h)
+        assert flips >= least and missed == 0, (flips, missed)
+
+
+def test_an_under_flagging_tangency_test_is_caught():
+    """Planted: a tangency margin 1000 times too small leaves silhouette flips
+    of the stress scene unflagged."""
+    from csgray import scenes as S
+    try:
+        O.set_tangent_eta(2e-9)
+        flips, missed = unflagged_flips(S.stress(4095, seed=4095), 640, 360)
+    finally:
+        O.set_tangent_eta(2e-6)
+    assert missed > 0
