@@ -49,14 +49,15 @@ class Outputs(ctypes.Structure):
 class SceneInfo(ctypes.Structure):
     _fields_ = [("nodes", ctypes.c_int32), ("primitives", ctypes.c_int32), ("op_depth", ctypes.c_int32),
                 ("spheres", ctypes.c_int32), ("cylinders", ctypes.c_int32), ("cubes", ctypes.c_int32),
-                ("device_bytes", ctypes.c_int64)]
+                ("device_bytes", ctypes.c_int64), ("prims_in_boxes", ctypes.c_int32)]
 
 
 class RenderOpts(ctypes.Structure):
     _fields_ = [("mode", ctypes.c_int32), ("block", ctypes.c_int32), ("scene_placement", ctypes.c_int32),
                 ("counters", ctypes.c_int32), ("counter_out", ctypes.c_void_p), ("tile_order", ctypes.c_void_p),
                 ("samples", ctypes.c_int32), ("seed", ctypes.c_uint32), ("tile_walk", ctypes.c_int32),
-                ("coop_budget", ctypes.c_int32), ("coop_when", ctypes.c_int32), ("coop_grace", ctypes.c_int32)]
+                ("coop_budget", ctypes.c_int32), ("coop_when", ctypes.c_int32), ("coop_grace", ctypes.c_int32),
+                ("coop_intervals", ctypes.c_int32)]
 
 
 class Job(ctypes.Structure):
@@ -133,6 +134,7 @@ _SIGNATURES = [
                                           ctypes.POINTER(RenderOpts), _P]),
     ("csg_sync", ctypes.c_int, [_P]),
     ("csg_coop_rays", ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int64)]),
+    ("csg_coop_interval_rays", ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int64)]),
     ("csg_camera_move", ctypes.c_int, [ctypes.POINTER(Camera), ctypes.c_float, ctypes.c_float, ctypes.c_float]),
     ("csg_camera_rotate", ctypes.c_int, [ctypes.POINTER(Camera), ctypes.c_float, ctypes.c_float]),
     ("csg_camera_input", ctypes.c_int, [ctypes.POINTER(Camera), ctypes.c_uint32, ctypes.c_int32, ctypes.c_float,
@@ -293,14 +295,15 @@ class Scene:
 
     def render(self, cam, light_, phong_, width, height, mode=MODE_FAST, outputs=None, counters=False,
                stream=None, device=None, block=0, placement=0, tile_order=None, samples=1, seed=0, walk=0, coop=0,
-               coop_when=0, coop_grace=0):
+               coop_when=0, coop_grace=0, coop_intervals=0):
         """Render into torch CUDA tensors; returns dict rgba/depth/prim_id/flags(/counters).
 
         `outputs` may pass preallocated tensors (same keys) to avoid allocation.  `coop` is the
         step budget after which a ray may move to the cooperative kernel (0 default, < 0 never);
         `coop_when` 0 moves it only in the frame's tail (once every tile has been taken and a further
         `coop_grace` percent of the time that took has passed; 0 default, < 0 none), 1 as soon as the
-        budget is spent."""
+        budget is spent.  `coop_intervals` < 0 keeps handed-over rays on the state machine (0: interval
+        classification where the scene allows, fast mode)."""
         import torch
         dev = device or torch.device("cuda", torch.cuda.current_device())
         spp = max(1, int(samples))
@@ -315,7 +318,7 @@ class Scene:
         opts = RenderOpts(mode, block, placement, 1 if counters else 0,
                           outputs["counters"].data_ptr() if counters else None,
                           tile_order.data_ptr() if tile_order is not None else None, spp, seed, walk, coop,
-                          coop_when, coop_grace)
+                          coop_when, coop_grace, coop_intervals)
         s = stream if stream is not None else torch.cuda.current_stream(dev).cuda_stream
         _check(lib().csg_render_ex(self.handle, ctypes.byref(cam), ctypes.byref(light_), ctypes.byref(phong_),
                                    int(width), int(height), ctypes.byref(o), ctypes.byref(opts), s))
@@ -399,6 +402,15 @@ def coop_rays(stream=None):
     return v.value
 
 
+def coop_interval_rays(stream=None):
+    """Of the handed-over rays, those finished by interval classification (waits for the stream)."""
+    import torch
+    s = stream if stream is not None else torch.cuda.current_stream().cuda_stream
+    v = ctypes.c_int64()
+    _check(lib().csg_coop_interval_rays(s, ctypes.byref(v)))
+    return v.value
+
+
 def jitter(seed, pixel, sample, axis):
     return float(lib().csg_jitter(seed, pixel, sample, axis))
 
@@ -418,10 +430,11 @@ def batch_order(jobs):
 
 
 def render_batch(jobs, mode=MODE_FAST, block=0, walk=0, samples=1, seed=0, stream=None, coop=0, coop_when=0,
-                 coop_grace=0):
+                 coop_grace=0, coop_intervals=0):
     """One persistent launch over a ctypes Job array (see make_jobs)."""
     import torch
-    opts = RenderOpts(mode, block, PLACE_GLOBAL, 0, None, None, samples, seed, walk, coop, coop_when, coop_grace)
+    opts = RenderOpts(mode, block, PLACE_GLOBAL, 0, None, None, samples, seed, walk, coop, coop_when, coop_grace,
+                      coop_intervals)
     s = stream if stream is not None else torch.cuda.current_stream().cuda_stream
     _check(lib().csg_render_batch(jobs, len(jobs), ctypes.byref(opts), s))
 
@@ -429,8 +442,10 @@ def render_batch(jobs, mode=MODE_FAST, block=0, walk=0, samples=1, seed=0, strea
 class Batch:
     """A captured batch (csg_batch*): jobs fixed at creation, graph replayed by launch()."""
 
-    def __init__(self, jobs, mode=MODE_FAST, block=0, walk=0, samples=1, seed=0, coop=0, coop_when=0, coop_grace=0):
-        opts = RenderOpts(mode, block, PLACE_GLOBAL, 0, None, None, samples, seed, walk, coop, coop_when, coop_grace)
+    def __init__(self, jobs, mode=MODE_FAST, block=0, walk=0, samples=1, seed=0, coop=0, coop_when=0, coop_grace=0,
+                 coop_intervals=0):
+        opts = RenderOpts(mode, block, PLACE_GLOBAL, 0, None, None, samples, seed, walk, coop, coop_when, coop_grace,
+                          coop_intervals)
         h = _P()
         _check(lib().csg_batch_create(jobs, len(jobs), ctypes.byref(opts), ctypes.byref(h)))
         self.handle = h

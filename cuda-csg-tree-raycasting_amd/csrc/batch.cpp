@@ -186,10 +186,12 @@ csg_status Stage(csg_batch* b) {
     if (cnt > ch.cap) return Fail(CSG_E_ARG, "csg_batch: chunk capacity exceeded");
     int64_t base = 0;
     int32_t depth = 1, maxOps = 1, maxLeaves = 1;
+    bool anyInBoxes = false;
     for (int32_t i = 0; i < cnt; i++) {
       const csg_job& j = b->jobs[plan.order[start + i]];
       maxOps = std::max(maxOps, j.scene->dev.nOps);
       maxLeaves = std::max(maxLeaves, j.scene->dev.opLeaves);
+      anyInBoxes |= j.scene->dev.conservative != 0;
       const csg_status st = BuildJob(j.scene, &j.camera, &j.light, &j.phong, j.width, j.height, &j.out, &b->opts,
                                      b->device, base, &ch.hostJobs[i]);
       if (st != CSG_OK) return st;
@@ -209,7 +211,7 @@ csg_status Stage(csg_batch* b) {
     if (e != hipSuccess) return FailHip(e, "choosing the launch shape");
     csgray::FitGridToTiles(&ch.shape, base);
     p.totalWaves = static_cast<uint32_t>(ch.shape.grid) * static_cast<uint32_t>(ch.shape.block / 64);
-    PrepareCoop(b->cfg, maxOps, maxLeaves, ch.heavy, ch.shape, &p);
+    PrepareCoop(b->cfg, maxOps, maxLeaves, anyInBoxes, ch.heavy, ch.shape, &p);
   }
   return CSG_OK;
 }

@@ -69,6 +69,7 @@ csg_status Upload(csg_scene* s) {
   s->dev.nPrims = f.nPrims;
   s->dev.opDepth = f.opDepth;
   s->dev.opLeaves = f.opLeaves;
+  s->dev.conservative = f.conservative ? 1u : 0u;
   return CSG_OK;
 }
 
@@ -132,10 +133,16 @@ struct CoopPlan {
   size_t ldsBytes = 0;
 };
 
-CoopPlan PlanCoop(const LaunchConfig& cfg, int32_t maxOps, int32_t maxLeaves) {
+// Interval classification applies to fast-mode launches without counters
+// that hold a scene whose primitives all lie in their boxes.
+bool CoopLists(const LaunchConfig& cfg, bool anyInBoxes) {
+  return cfg.coopIntervals && cfg.mode == Mode::kFast && !cfg.count && anyInBoxes;
+}
+
+CoopPlan PlanCoop(const LaunchConfig& cfg, int32_t maxOps, int32_t maxLeaves, bool lists) {
   CoopPlan c;
   if (cfg.budget < 0 || maxOps <= 0) return c;
-  const size_t wave = CoopLdsBytes(maxOps, maxLeaves, &c.queueCap);
+  const size_t wave = CoopLdsBytes(maxOps, maxLeaves, lists, &c.queueCap);
   if (wave == 0) return c;
   int cus = 0;
   size_t ldsPerBlock = 0, ldsPerCu = 0;
@@ -166,11 +173,12 @@ int32_t DefaultCoopGraceMille(int64_t waves, int64_t tiles, bool batched) {
   return static_cast<int32_t>(std::min<int64_t>(std::max<int64_t>(g, 5), batched ? 1000000 : 3000));
 }
 
-void PrepareCoop(const LaunchConfig& cfg, int32_t maxOps, int32_t maxLeaves, uint2* heavy, const LaunchShape& shape,
-                 RenderParams* p) {
+void PrepareCoop(const LaunchConfig& cfg, int32_t maxOps, int32_t maxLeaves, bool anyInBoxes, uint2* heavy,
+                 const LaunchShape& shape, RenderParams* p) {
   p->budget = 0;
   if (!heavy) return;
-  const CoopPlan c = PlanCoop(cfg, maxOps, maxLeaves);
+  const bool lists = CoopLists(cfg, anyInBoxes);
+  const CoopPlan c = PlanCoop(cfg, maxOps, maxLeaves, lists);
   if (c.budget <= 0) return;
   p->budget = c.budget;
   p->heavy = heavy;
@@ -178,6 +186,7 @@ void PrepareCoop(const LaunchConfig& cfg, int32_t maxOps, int32_t maxLeaves, uin
   p->resume = reinterpret_cast<uint32_t*>(heavy + kHeavyCap);
   p->resumeBytes = static_cast<uint32_t>(kResumeBytes);
   p->coopOps = maxOps;
+  p->coopLists = lists ? 1 : 0;
   p->coopQueueCap = c.queueCap;
   p->coopWaves = c.waves;
   p->coopEager = cfg.coopEager ? 1 : 0;
@@ -287,6 +296,7 @@ csg_status csg_scene_info_get(const csg_scene* s, csg_scene_info* info) {
     else info->cubes++;
   }
   info->device_bytes = static_cast<int64_t>(s->bytes);
+  info->prims_in_boxes = s->flat.conservative ? 1 : 0;
   return CSG_OK;
 }
 
@@ -399,6 +409,7 @@ csg_status ParseOpts(const csg_render_opts* opts, csgray::LaunchConfig* cfg) {
   cfg->coopEager = opts->coop_when == 1;
   if (opts->coop_grace > 100000) return Fail(CSG_E_ARG, "coop_grace above 100000 percent");
   cfg->coopGrace = opts->coop_grace;
+  cfg->coopIntervals = opts->coop_intervals >= 0;
   return CSG_OK;
 }
 
@@ -570,7 +581,7 @@ csg_status csg_render_plan(const csg_scene* s, const csg_render_opts* opts, csg_
   info->cache_ops = shape.cacheOps;
   info->compute_units = cus;
   info->lds_limit = static_cast<int64_t>(ldsLimit);
-  const CoopPlan c = PlanCoop(cfg, s->dev.nOps, s->dev.opLeaves);
+  const CoopPlan c = PlanCoop(cfg, s->dev.nOps, s->dev.opLeaves, CoopLists(cfg, s->dev.conservative != 0));
   info->coop_budget = c.budget;
   info->coop_grid = static_cast<int32_t>(c.waves);
   info->coop_block = 64;
@@ -584,8 +595,9 @@ csg_status csg_render_plan(const csg_scene* s, const csg_render_opts* opts, csg_
   return CSG_OK;
 }
 
-csg_status csg_coop_rays(void* stream, int64_t* rays) {
-  if (!rays) return Fail(CSG_E_ARG, "csg_coop_rays: NULL argument");
+namespace {
+// Reads a 64-bit running total of the stream's queue words after waiting for the stream.
+csg_status QueueTotal(void* stream, uint32_t word, int64_t* value) {
   int dev = 0;
   hipError_t e = hipGetDevice(&dev);
   if (e != hipSuccess) return FailHip(e, "hipGetDevice");
@@ -596,10 +608,21 @@ csg_status csg_coop_rays(void* stream, int64_t* rays) {
   e = hipStreamSynchronize(hs);
   if (e != hipSuccess) return FailHip(e, "hipStreamSynchronize");
   unsigned long long v = 0;
-  e = hipMemcpy(&v, q + 6, sizeof v, hipMemcpyDeviceToHost);
+  e = hipMemcpy(&v, q + word, sizeof v, hipMemcpyDeviceToHost);
   if (e != hipSuccess) return FailHip(e, "hipMemcpy");
-  *rays = static_cast<int64_t>(v);
+  *value = static_cast<int64_t>(v);
   return CSG_OK;
+}
+}  // namespace
+
+csg_status csg_coop_interval_rays(void* stream, int64_t* rays) {
+  if (!rays) return Fail(CSG_E_ARG, "csg_coop_interval_rays: NULL argument");
+  return QueueTotal(stream, 8, rays);
+}
+
+csg_status csg_coop_rays(void* stream, int64_t* rays) {
+  if (!rays) return Fail(CSG_E_ARG, "csg_coop_rays: NULL argument");
+  return QueueTotal(stream, 6, rays);
 }
 
 float csg_jitter(uint32_t seed, uint32_t pixel, uint32_t sample, uint32_t axis) {
@@ -639,7 +662,7 @@ csg_status csg_render_ex(const csg_scene* s, const csg_camera* cam, const csg_li
     const csg_status hsd = HeavyFor(dev, hs, &heavy);
     if (hsd != CSG_OK) return hsd;
   }
-  PrepareCoop(cfg, s->dev.nOps, s->dev.opLeaves, heavy, shape, &p);
+  PrepareCoop(cfg, s->dev.nOps, s->dev.opLeaves, s->dev.conservative != 0, heavy, shape, &p);
   e = csgray::LaunchRender(p, cfg, shape, hs);
   if (e != hipSuccess) return FailHip(e, "render launch");
   return CSG_OK;
@@ -709,11 +732,13 @@ csg_status csg_render_batch(const csg_job* jobs, int32_t njobs, const csg_render
       if (hsd != CSG_OK) return hsd;
     }
     int32_t maxOps = 1, maxLeaves = 1;
+    bool anyInBoxes = false;
     for (int32_t i = 0; i < n; i++) {
       maxOps = std::max(maxOps, jobs[start + i].scene->dev.nOps);
       maxLeaves = std::max(maxLeaves, jobs[start + i].scene->dev.opLeaves);
+      anyInBoxes |= jobs[start + i].scene->dev.conservative != 0;
     }
-    PrepareCoop(cfg, maxOps, maxLeaves, heavy, shape, &p);
+    PrepareCoop(cfg, maxOps, maxLeaves, anyInBoxes, heavy, shape, &p);
     e = csgray::LaunchRender(p, cfg, shape, hs);
     if (e != hipSuccess) return FailHip(e, "batched render launch");
   }

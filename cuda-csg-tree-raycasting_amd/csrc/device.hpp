@@ -19,6 +19,7 @@ struct DevScene {
   int32_t nPrims;
   int32_t opDepth;
   int32_t opLeaves;  // operators without an operator child: a ray's live cooperative tasks never exceed it
+  uint32_t conservative;  // 1: every primitive lies inside its box (the cooperative list evaluation applies)
 };
 
 // One render job: a scene seen by one camera at one resolution, with its
@@ -67,6 +68,7 @@ struct RenderParams {
   uint2* heavy;
   uint32_t heavyCap;
   int32_t coopOps;            // operator-state slots per wave (max operators of the launch)
+  int32_t coopLists;          // 1: the region holds interval classification's view too (fast mode)
   uint32_t coopQueueCap;      // task ring entries (power of two >= the launch's largest opLeaves)
   uint32_t coopWaves;         // waves in the cooperative grid
   // Resume area (`resumeBytes`, right after the heavy list): for the first
@@ -77,7 +79,7 @@ struct RenderParams {
   int32_t cacheOps;           // plain global-memory launches: operator records [0, cacheOps) staged in LDS
 };
 // queue words: [1] render workgroups done, [2] heavy rays, [4] coop waves done, [6..7] heavy rays handed
-// over on this stream so far (64-bit); on their own cache line, [32..33] time the first tile was taken,
+// over on this stream so far (64-bit), [8..9] of those, the ones finished from boundary lists; on their own cache line, [32..33] time the first tile was taken,
 // [34..35] time the tiles ran out (64-bit wall clock | 1, 0 = not yet).  Tiles and heavy rays are taken
 // from kQueueStripes counters each, 256 B apart: stripe s hands out items s, s + kQueueStripes, ...
 // (counter = items of the stripe taken so far), so the waves' atomics spread over many addresses
@@ -112,6 +114,7 @@ struct LaunchConfig {
   int32_t budget = 0;       // 0: default step budget, < 0: no deferral, > 0: steps before deferral
   bool coopEager = false;   // hand rays over as soon as the budget is spent, not only after the queue drains
   int32_t coopGrace = 0;    // 0: default; < 0: none; else percent (see RenderParams::coopGrace)
+  bool coopIntervals = true;  // interval classification of heavy rays where it applies
 };
 
 // Launch geometry chosen for a scene: workgroup size, grid, LDS bytes.
@@ -166,12 +169,16 @@ constexpr uint32_t kResumeFlag = 1u << 31;
 constexpr size_t kHeavyBytes = sizeof(uint32_t) * 2 * kHeavyCap + kResumeBytes;
 
 // LDS bytes the cooperative kernel needs for scenes of at most `ops`
-// operators of which at most `leaves` have no operator child (0 when it
-// cannot run: the caller then disables deferral).
-size_t CoopLdsBytes(int32_t ops, int32_t leaves, uint32_t* queueCap);
-// LDS of one wave's cooperative state (one state per operator, the task
-// ring of 16-bit tasks and their float tmin; rounded to 16 B), i.e. of one
-// cooperative workgroup of `p`.
+// operators of which at most `leaves` have no operator child, with room for
+// interval classification when `lists` (0 when it cannot run: the caller
+// then disables deferral).
+size_t CoopLdsBytes(int32_t ops, int32_t leaves, bool lists, uint32_t* queueCap);
+// LDS of one wave's cooperative state, i.e. of one cooperative workgroup,
+// rounded to 16 B.  Two views of the same bytes: the state machine's (one
+// 16-byte state per operator, then the task ring's float tmins) and, when
+// `lists`, interval classification's (per operator two 4-byte child masks,
+// a 4-byte word and a 2-byte ready slot; then the crossings, 12 bytes each);
+// the ring's 16-bit tasks end the region in both.
 #if defined(__HIPCC__)
 #define CSG_HOST_DEVICE __host__ __device__
 #else
@@ -179,9 +186,16 @@ size_t CoopLdsBytes(int32_t ops, int32_t leaves, uint32_t* queueCap);
 #endif
 constexpr size_t kCoopOpStateBytes = 16;
 constexpr size_t kCoopTaskBytes = 6;
-CSG_HOST_DEVICE inline size_t CoopWaveLds(int32_t ops, uint32_t queueCap) {
-  const size_t bytes =
+constexpr uint32_t kCoopMaxCross = 31;  // crossings a ray's interval masks (32 bits) can order
+CSG_HOST_DEVICE inline size_t CoopCrossOffset(int32_t ops) {
+  return (static_cast<size_t>(ops) * 14 + 3) & ~static_cast<size_t>(3);
+}
+CSG_HOST_DEVICE inline size_t CoopWaveLds(int32_t ops, uint32_t queueCap, int32_t lists) {
+  const size_t machine =
       static_cast<size_t>(ops) * kCoopOpStateBytes + static_cast<size_t>(queueCap) * kCoopTaskBytes;
+  const size_t masks = CoopCrossOffset(ops) + 12 * (static_cast<size_t>(kCoopMaxCross) + 1) +
+                       static_cast<size_t>(queueCap) * 2;
+  const size_t bytes = lists && masks > machine ? masks : machine;
   return (bytes + 15) & ~static_cast<size_t>(15);
 }
 size_t CoopLaunchLds(const RenderParams& p);

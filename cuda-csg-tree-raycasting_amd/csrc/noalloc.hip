@@ -112,7 +112,8 @@ extern "C" csg_status csg_render_noalloc(const csg_scene* s, const csg_camera* c
   if (ss != CSG_OK) return ss;
   p.queue = reinterpret_cast<uint32_t*>(base);
   uint2* heavy = reinterpret_cast<uint2*>(base + kQueueRegion);
-  PrepareCoop(cfg, s->dev.nOps, s->dev.opLeaves, cfg.budget >= 0 ? heavy : nullptr, shape, &p);
+  PrepareCoop(cfg, s->dev.nOps, s->dev.opLeaves, s->dev.conservative != 0, cfg.budget >= 0 ? heavy : nullptr, shape,
+              &p);
   p.resumeBytes = static_cast<uint32_t>(kNoAllocResumeBytes);
   e = csgray::LaunchRender(p, cfg, shape, hs);
   if (e != hipSuccess) return FailHip(e, "render launch");

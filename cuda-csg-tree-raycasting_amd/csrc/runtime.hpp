@@ -45,8 +45,8 @@ csg_status HeavyFor(int dev, hipStream_t stream, uint2** out);
 // Fills the cooperative-evaluation fields of `p` for a launch whose scenes
 // have at most `maxOps` operators (deferral off when the options say so or
 // the cooperative kernel's state does not fit LDS).
-void PrepareCoop(const LaunchConfig& cfg, int32_t maxOps, int32_t maxLeaves, uint2* heavy, const LaunchShape& shape,
-                 RenderParams* p);
+void PrepareCoop(const LaunchConfig& cfg, int32_t maxOps, int32_t maxLeaves, bool anyInBoxes, uint2* heavy,
+                 const LaunchShape& shape, RenderParams* p);
 
 // Copies a job table to the (device, stream) table buffer in stream order.
 csg_status UploadJobs(int dev, hipStream_t stream, const Job* jobs, size_t n, const Job** out);

@@ -116,6 +116,20 @@ Box LeafBox(const Primitive& p) {  // BVHNode.cuh:17-65
   return b;
 }
 
+// Whether the cylinder lies inside its box: the box's half-extent max(h/2, r)
+// against the true one along each axis, |V_k| h/2 + r sqrt(1 - V_k^2), up to
+// 1e-6 relative (an axis-aligned cylinder fits its box exactly).  Spheres
+// (half-extent 2r) and cubes (their own faces) always do.
+bool CylinderInBox(const Primitive& p) {
+  const double e = std::max(p.height / 2, p.radius);
+  for (int k = 0; k < 3; k++) {
+    const double v = p.axis[k];
+    const double ext = std::fabs(v) * (p.height / 2.0) + p.radius * std::sqrt(std::max(0.0, 1.0 - v * v));
+    if (ext > e * (1.0 + 1e-6)) return false;
+  }
+  return true;
+}
+
 }  // namespace
 
 CSGTree CSGTree::Parse(const std::string& text) {
@@ -287,6 +301,7 @@ FlatScene Flatten(const CSGTree& tree) {
     f.primA[p] = F4{q.x, q.y, q.z, s};
     f.primB[p] = F4{q.axis[0], q.axis[1], q.axis[2], q.height};
     f.primC[p] = F4{q.r, q.g, q.b, 0.0f};
+    if (q.type == kCylinder && !CylinderInBox(q)) f.conservative = false;
   }
   return f;
 }

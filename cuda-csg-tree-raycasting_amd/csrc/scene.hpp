@@ -89,6 +89,7 @@ struct FlatScene {
   uint32_t rootDesc = 0;
   int32_t nNodes = 0, nOps = 0, nPrims = 0, opDepth = 0;
   int32_t opLeaves = 0;  // operators without an operator child
+  bool conservative = true;  // every primitive lies inside its box (false: a tilted cylinder pokes out)
 };
 
 constexpr int32_t kMaxNodes = 65535;       // 16-bit node links

@@ -180,6 +180,7 @@ void Dispatch(csg_server* s, Slot& sl) {
   jobs.reserve(take.size());
   int64_t base = 0;
   int32_t depth = 1, maxOps = 1, maxLeaves = 1;
+  bool anyInBoxes = false;
   csg_render_opts opts{};
   opts.mode = s->cfg.mode;
   bool failed = false;
@@ -213,6 +214,7 @@ void Dispatch(csg_server* s, Slot& sl) {
     depth = std::max(depth, sc->dev.opDepth);
     maxOps = std::max(maxOps, sc->dev.nOps);
     maxLeaves = std::max(maxLeaves, sc->dev.opLeaves);
+    anyInBoxes |= sc->dev.conservative != 0;
     r.state = kRunning;
   }
   if (failed) s->doneCv.notify_all();
@@ -235,7 +237,7 @@ void Dispatch(csg_server* s, Slot& sl) {
     p.queue = queue;
     csgray::FitGridToTiles(&shape, base);
     p.totalWaves = static_cast<uint32_t>(shape.grid) * static_cast<uint32_t>(shape.block / 64);
-    PrepareCoop(s->launch, maxOps, maxLeaves, heavy, shape, &p);
+    PrepareCoop(s->launch, maxOps, maxLeaves, anyInBoxes, heavy, shape, &p);
     if (csgray::LaunchRender(p, s->launch, shape, sl.stream) != hipSuccess) st = CSG_E_HIP;
   }
   if (st == CSG_OK && hipEventRecord(sl.done, sl.stream) != hipSuccess) st = CSG_E_HIP;

@@ -90,6 +90,9 @@ typedef struct csg_scene_info {
   int32_t op_depth;  /* operator levels above the deepest leaf */
   int32_t spheres, cylinders, cubes;
   int64_t device_bytes;
+  int32_t prims_in_boxes;  /* 1: every primitive lies inside its box (a cylinder tilted off the
+                              coordinate axes may not: the reference's box is centre +- max(h/2, r)),
+                              so handed-over rays may be classified by intervals */
 } csg_scene_info;
 
 /* Arithmetic of the traversal kernel.
@@ -129,6 +132,11 @@ typedef struct csg_render_opts {
                            a batch (the tiles in flight when the queue runs dry finish in about a tile's
                            time; handing out took tiles / waves tile times): 120 for a 1024-lane launch of
                            1920x1080, about 3 for the 256-tree batch; < 0 = none; else percent */
+  int32_t coop_intervals; /* 0 = default: in fast mode, a handed-over ray of a scene whose primitives all
+                             lie inside their boxes is classified by intervals (one pass over the hit part
+                             of the tree, the crossings sorted, membership masks combined bottom-up; the
+                             state machine's result, which a ray gets whenever the two could order events
+                             differently); < 0 = state machine only */
 } csg_render_opts;
 
 /* --- scene (replaces CSGTree::Parse + the upload in Raycaster::ChangeSize) --- */
@@ -213,6 +221,10 @@ csg_status csg_sync(void* stream);
  * and csg_render_batch launches on `stream` (current device); waits for the
  * stream.  No reference counterpart: for tuning the hand-over and for tests. */
 csg_status csg_coop_rays(void* stream, int64_t* rays);
+/* Of those, the rays the cooperative kernel finished by interval
+ * classification (csg_render_opts.coop_intervals) rather than with the state
+ * machine. */
+csg_status csg_coop_interval_rays(void* stream, int64_t* rays);
 
 /* Sub-pixel offset in [0,1) of sample `sample` of pixel `pixel` along axis 0
  * (x) or 1 (y): a 32-bit integer hash of (seed, pixel, sample, axis) with 24

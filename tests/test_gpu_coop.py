@@ -19,6 +19,7 @@ import oracle as O
 from common import setups
 from conftest import scene_names, scene_text
 from csgray import scenes as S
+import deep_trees as deep
 
 pytestmark = pytest.mark.gpu
 
@@ -87,6 +88,61 @@ def test_fast_mode_same_with_and_without_deferral(gpu):
     for budget, when in ((1, 1), (64, 1), (64, 0), (0, 0)):
         _same(_np(s.render(cam, light, phong, 1920, 1080, coop=budget, coop_when=when)), ref)
     s.close()
+
+
+def _three_ways(s, cam, light, phong, w, h, **kw):
+    """Per-lane only; every ray handed over (budget 1, eager) with interval
+    classification; the same with the state machine.  Returns the three
+    outputs and the rays the second render handed over and classified."""
+    lane = _np(s.render(cam, light, phong, w, h, coop=-1, **kw))
+    before, handed0 = C.coop_interval_rays(), C.coop_rays()
+    iv = _np(s.render(cam, light, phong, w, h, coop=1, coop_when=1, **kw))
+    classified, handed = C.coop_interval_rays() - before, C.coop_rays() - handed0
+    machine = _np(s.render(cam, light, phong, w, h, coop=1, coop_when=1, coop_intervals=-1, **kw))
+    return lane, iv, machine, classified, handed
+
+
+@pytest.mark.parametrize("setup", [0, 1])
+@pytest.mark.parametrize("name", scene_names())
+def test_interval_classification_gives_the_state_machines_bits(gpu, name, setup):
+    """Fast mode with every ray handed over: interval classification, the
+    state-machine wave and the per-lane kernel give the same bits in all
+    four outputs on every reference scene (coplanar faces and shared edges
+    included: rays whose crossings tie go to the state machine), and scenes
+    whose primitives all lie in their boxes are classified."""
+    print(f"\n[case] {name} setup={setup} fast, intervals vs state machine", flush=True)
+    sname, cam, light, phong, cam6, l3, ph8 = setups()[setup]
+    s = C.Scene(scene_text(name))
+    lane, iv, machine, classified, handed = _three_ways(s, cam, light, phong, 160, 90)
+    inside = s.info().prims_in_boxes
+    s.close()
+    print(f"handed={handed} classified={classified} prims_in_boxes={inside}")
+    _same(iv, lane)
+    _same(machine, lane)
+    if inside and handed > 0:
+        assert classified > 0
+    if not inside:
+        assert classified == 0
+
+
+@pytest.mark.parametrize("make", [lambda: S.cheese(512, seed=512), lambda: S.cheese(64, seed=3, r_lo=0.2, r_hi=0.5),
+                                  lambda: deep.left_deep_difference(40, seed=2),
+                                  lambda: deep.mixed_unbalanced(24, seed=5, cylinders=False)],
+                         ids=["cheese512", "cheese64_big", "difference_chain40", "mixed24"])
+def test_interval_classification_on_heavy_scenes(gpu, make):
+    """The same three-way identity on scenes with long loops (the benchmark
+    scene at 960x540 with 4 jittered samples, a coarse cheese, a 40-level
+    Difference chain, a 24-level mixed tree); most handed-over rays are
+    classified."""
+    print("\n[case] heavy scene fast, intervals vs state machine, 960x540x4", flush=True)
+    sname, cam, light, phong, cam6, l3, ph8 = setups()[1]
+    s = C.Scene(make())
+    lane, iv, machine, classified, handed = _three_ways(s, cam, light, phong, 960, 540, samples=4, seed=9)
+    s.close()
+    print(f"handed={handed} classified={classified}")
+    _same(iv, lane)
+    _same(machine, lane)
+    assert handed > 0 and classified > 0.9 * handed
 
 
 def test_heavy_list_overflow_finishes_per_lane(gpu):

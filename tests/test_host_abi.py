@@ -100,6 +100,20 @@ def test_scene_info_counts():
     assert i.device_bytes == 0
 
 
+@pytest.mark.parametrize("text,inside", [
+    ("Sphere 0 0 0 FF0000 1", 1),
+    ("Cube 0 0 0 FF0000 1", 1),
+    ("Cylinder 0 0 0 FF0000 1 2 0 0 0", 1),      # axis y: half-extents (r, h/2, r) = the box's 1
+    ("Cylinder 0 0 0 FF0000 1 2 90 0 0", 1),     # axis z
+    ("Cylinder 0 0 0 FF0000 0.1 4 30 0 0", 1),   # long and thin: inside the h/2 cube however tilted
+    ("Cylinder 0 0 0 FF0000 1 2 45 0 0", 0),     # y and z half-extents sqrt(2) > 1
+    ("Union\n Sphere 0 0 0 FF0000 1\n Cylinder 3 0 0 FF0000 1 1 0 0 30\n", 0),
+])
+def test_scene_info_reports_primitives_inside_their_boxes(text, inside):
+    s = C.Scene(text, upload=False)
+    assert s.info().prims_in_boxes == inside
+
+
 def test_render_refuses_host_only_scene_without_touching_the_gpu():
     s = C.Scene(scene_text("test1.txt"), upload=False)
     o = C.Outputs(None, None, None, None)
