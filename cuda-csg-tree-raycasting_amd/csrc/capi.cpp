@@ -148,7 +148,7 @@ CoopPlan PlanCoop(const LaunchConfig& cfg, int32_t maxOps, int32_t maxLeaves, bo
   size_t ldsPerBlock = 0, ldsPerCu = 0;
   if (DeviceShapeLimits(&cus, &ldsPerBlock, &ldsPerCu) != hipSuccess || cus <= 0) return c;
   const size_t perCu = std::max<size_t>(1, std::min<size_t>(16, ldsPerCu / (wave + 256)));
-  c.budget = cfg.budget > 0 ? cfg.budget : kDefaultBudget;
+  c.budget = cfg.budget > 0 ? cfg.budget : (lists ? kDefaultIntervalBudget : kDefaultBudget);
   c.waves = static_cast<uint32_t>(static_cast<size_t>(cus) * perCu);
   c.ldsBytes = wave;
   return c;
@@ -165,10 +165,15 @@ CoopPlan PlanCoop(const LaunchConfig& cfg, int32_t maxOps, int32_t maxLeaves, bo
 // tail (the last tile goes out after one or two tile times), so a single
 // scene's grace stays at or below 300 %: the 256 random trees of the
 // batched mix, each in its own launch, 222 -> 215.5 ms, the cheese frames
-// unchanged (a cap of 150 % would cost its 1024x1024 frame 10 %).
-int32_t DefaultCoopGraceMille(int64_t waves, int64_t tiles, bool batched) {
+// unchanged (a cap of 150 % would cost its 1024x1024 frame 10 %).  Where
+// the cooperative kernel classifies rays by intervals (`lists`), a handed-
+// over ray costs it about half as much, and the tail is best handed over
+// sooner: k = 5500 for single scenes too (cheese frames, measured best:
+// 60-80 % at 1920x1080, 20 % or less at 3840x2160, 100 % or more at
+// 1024x1024; k = 5500 gives 70, 17 and 137 %).
+int32_t DefaultCoopGraceMille(int64_t waves, int64_t tiles, bool batched, bool lists) {
   if (tiles <= 0) return 1000;
-  const int64_t k = batched ? 5500 : 9500;
+  const int64_t k = (batched || lists) ? 5500 : 9500;
   const int64_t g = (k * waves + tiles / 2) / tiles;
   return static_cast<int32_t>(std::min<int64_t>(std::max<int64_t>(g, 5), batched ? 1000000 : 3000));
 }
@@ -191,7 +196,9 @@ void PrepareCoop(const LaunchConfig& cfg, int32_t maxOps, int32_t maxLeaves, boo
   p->coopWaves = c.waves;
   p->coopEager = cfg.coopEager ? 1 : 0;
   p->coopGrace = cfg.coopGrace > 0 ? cfg.coopGrace * 10
-                                   : (cfg.coopGrace < 0 ? 0 : DefaultCoopGraceMille(p->totalWaves, p->numTiles, p->numJobs > 0));
+                                   : (cfg.coopGrace < 0 ? 0
+                                                        : DefaultCoopGraceMille(p->totalWaves, p->numTiles,
+                                                                                p->numJobs > 0, lists));
 }
 
 }  // namespace rt
@@ -581,7 +588,8 @@ csg_status csg_render_plan(const csg_scene* s, const csg_render_opts* opts, csg_
   info->cache_ops = shape.cacheOps;
   info->compute_units = cus;
   info->lds_limit = static_cast<int64_t>(ldsLimit);
-  const CoopPlan c = PlanCoop(cfg, s->dev.nOps, s->dev.opLeaves, CoopLists(cfg, s->dev.conservative != 0));
+  const bool lists = CoopLists(cfg, s->dev.conservative != 0);
+  const CoopPlan c = PlanCoop(cfg, s->dev.nOps, s->dev.opLeaves, lists);
   info->coop_budget = c.budget;
   info->coop_grid = static_cast<int32_t>(c.waves);
   info->coop_block = 64;
@@ -589,7 +597,9 @@ csg_status csg_render_plan(const csg_scene* s, const csg_render_opts* opts, csg_
   const int64_t hdTiles = 240 * 135;
   const int64_t waves = int64_t{shape.grid} * (shape.block / 64);
   info->coop_grace = c.budget > 0 ? (cfg.coopGrace > 0 ? cfg.coopGrace
-                                                       : (cfg.coopGrace < 0 ? 0 : (DefaultCoopGraceMille(waves, hdTiles, false) + 5) / 10))
+                                                       : (cfg.coopGrace < 0 ? 0
+                                                                            : (DefaultCoopGraceMille(waves, hdTiles, false,
+                                                                                                     lists) + 5) / 10))
                                   : 0;
   info->coop_lds_bytes = static_cast<int64_t>(c.ldsBytes);
   return CSG_OK;

@@ -225,9 +225,9 @@ __device__ __forceinline__ bool BoxHit(const RayF& r, float lx, float ly, float 
 // cancellation of two numbers of the size |oc|^2, so its error scales with
 // the sphere and not with its distance.  It also takes the hardware square
 // root (about 1 ulp).
+// The sphere's roots -b -+ sq; false when the ray misses it.
 template <bool kFast>
-__device__ __forceinline__ Hit SphereHit(const RayF& r, const float4& a, uint32_t prim, float tmin, Hit h) {
-  h.bits = HitBits(kMissBit, 0, prim);
+__device__ __forceinline__ bool SphereRoots(const RayF& r, const float4& a, float& t0, float& t1) {
   const float ocx = r.ox - a.x, ocy = r.oy - a.y, ocz = r.oz - a.z;
   const float b = ocx * r.dx + ocy * r.dy + ocz * r.dz;
   float disc;
@@ -238,18 +238,34 @@ __device__ __forceinline__ Hit SphereHit(const RayF& r, const float4& a, uint32_
     const float c = (ocx * ocx + ocy * ocy + ocz * ocz) - a.w * a.w;
     disc = b * b - c;
   }
-  if (disc < 0) return h;
+  if (disc < 0) return false;
   const float sq = kFast ? __builtin_amdgcn_sqrtf(disc) : sqrtf(disc);
-  float temp = (-b - sq);
-  if (temp <= tmin) {
-    temp = (-b + sq);
-    if (temp <= tmin) return Hit{-1.0f, HitBits(kMissBit, 0, 0xFFFFu)};
-  }
+  t0 = (-b - sq);
+  t1 = (-b + sq);
+  return true;
+}
+
+// The sphere's hit record at root `temp` (Enter where the outward normal
+// faces the ray).
+__device__ __forceinline__ Hit SphereAt(const RayF& r, const float4& a, uint32_t prim, float temp) {
   const float nx = (r.ox + temp * r.dx) - a.x;
   const float ny = (r.oy + temp * r.dy) - a.y;
   const float nz = (r.oz + temp * r.dz) - a.z;
   const float dn = nx * r.dx + ny * r.dy + nz * r.dz;
   return Hit{temp, HitBits(dn <= 0 ? kEnter : kExit, kSphereT, prim)};
+}
+
+template <bool kFast>
+__device__ __forceinline__ Hit SphereHit(const RayF& r, const float4& a, uint32_t prim, float tmin, Hit h) {
+  h.bits = HitBits(kMissBit, 0, prim);
+  float t0, t1;
+  if (!SphereRoots<kFast>(r, a, t0, t1)) return h;
+  float temp = t0;
+  if (temp <= tmin) {
+    temp = t1;
+    if (temp <= tmin) return Hit{-1.0f, HitBits(kMissBit, 0, 0xFFFFu)};
+  }
+  return SphereAt(r, a, prim, temp);
 }
 
 // cylinderHit (:202-336).  Both roots go through the same cap logic, as in
@@ -364,27 +380,38 @@ __device__ __forceinline__ Hit CylinderHit(const RayF& r, const float4& a, const
 }
 
 // cubeHit (:375-434).
+// The cube's slab limits; false when the ray misses it.
 template <bool kFast>
-__device__ __forceinline__ Hit CubeHit(const RayF& r, const float4& a, uint32_t prim, float tmin, Hit h) {
-  h.bits = HitBits(kMissBit, 0, prim);
+__device__ __forceinline__ bool CubeSlab(const RayF& r, const float4& a, float& tmn, float& tmx) {
   const float half = a.w / 2;
   const float lx = a.x - half * 1.0f, ly = a.y - half * 1.0f, lz = a.z - half * 1.0f;
   const float hx = a.x + half * 1.0f, hy = a.y + half * 1.0f, hz = a.z + half * 1.0f;
-  float tmn, tmx;
   Slab<kFast>(r, lx, ly, lz, hx, hy, hz, tmn, tmx);
-  if (tmx < 0) return h;
-  if (tmn > tmx) return h;
+  return !(tmx < 0) && !(tmn > tmx);
+}
+
+// The cube's hit record at slab limit `t` (the face normal by truncation).
+__device__ __forceinline__ Hit CubeAt(const RayF& r, const float4& a, uint32_t prim, float t) {
+  const float half = a.w / 2;
+  const float px = (r.ox + t * r.dx) - a.x;
+  const float py = (r.oy + t * r.dy) - a.y;
+  const float pz = (r.oz + t * r.dz) - a.z;
+  const float bias = 1.00001f;
+  const float nx = TruncInt(px / half * bias), ny = TruncInt(py / half * bias), nz = TruncInt(pz / half * bias);
+  const float dn = nx * r.dx + ny * r.dy + nz * r.dz;
+  return Hit{t, HitBits(dn <= 0 ? kEnter : kExit, kCubeT, prim)};
+}
+
+template <bool kFast>
+__device__ __forceinline__ Hit CubeHit(const RayF& r, const float4& a, uint32_t prim, float tmin, Hit h) {
+  h.bits = HitBits(kMissBit, 0, prim);
+  float tmn, tmx;
+  if (!CubeSlab<kFast>(r, a, tmn, tmx)) return h;
   if (tmn <= tmin) {
     tmn = tmx;
     if (tmn <= tmin) return h;
   }
-  const float px = (r.ox + tmn * r.dx) - a.x;
-  const float py = (r.oy + tmn * r.dy) - a.y;
-  const float pz = (r.oz + tmn * r.dz) - a.z;
-  const float bias = 1.00001f;
-  const float nx = TruncInt(px / half * bias), ny = TruncInt(py / half * bias), nz = TruncInt(pz / half * bias);
-  const float dn = nx * r.dx + ny * r.dy + nz * r.dz;
-  return Hit{tmn, HitBits(dn <= 0 ? kEnter : kExit, kCubeT, prim)};
+  return CubeAt(r, a, prim, tmn);
 }
 
 struct Counts {
@@ -1362,9 +1389,9 @@ __device__ Hit CoopTrace(const SceneView& sv, const RayF& r, OpState* st, uint16
 //      the last), and each primitive's membership over the intervals becomes
 //      a 32-bit mask (bits after its Enter, up to its Exit);
 //   C. the operators combine their children's masks bottom-up (Union: or,
-//      Intersection: and, Difference: and-not), each operator as soon as both
-//      children are done, a lane going on up the tree while its delivery
-//      completes the parent.
+//      Intersection: and, Difference: and-not), all operators entered in one
+//      walk iteration at once, from the last iteration back (a child is
+//      entered after its parent, so its mask is ready first).
 // The answer is the first crossing at which the root's mask changes; its hit
 // bits are the primitive's, with Enter and Exit swapped and Flip toggled once
 // per Difference whose right subtree holds it (the state machine's flip of a
@@ -1386,115 +1413,181 @@ __device__ Hit CoopTrace(const SceneView& sv, const RayF& r, OpState* st, uint16
 //
 // LDS (the cooperative kernel's per-wave region, CoopWaveLds): per operator
 // two child masks (8 B) and a word: parent (bits 0-14, 0x7FFF for the root),
-// operator (15-16), right child of its parent (17), child lists to come
-// (24-25); then one 16-bit ready slot per operator, the crossings (time, hit
-// bits, owner: operator << 2 | side << 1 | flip parity), and the task ring
+// operator (15-16), right child of its parent (17), the walk iteration that
+// entered it (18-31); then the entered operators in order (16 bits each), the
+// crossings (time, hit bits, owner: operator << 2 | side << 1 | flip parity),
+// one iteration's primitive tests (owner, descriptor), and the task ring
 // (ENTER tasks: operator | flip parity << 15) at the region's end.
 constexpr uint32_t kOpUnion = 0, kOpDifference = 1;  // operator numbering (CSGTree.cuh:38-46)
 constexpr uint32_t kIvFlip = kFlip | kExit | kEnter;
-constexpr uint32_t kIvRoot = 0x7FFFu, kIvPendShift = 24, kIvPendOne = 1u << kIvPendShift;
-constexpr int kIvPasses = 66;  // operator levels a lane climbs in one iteration (64 at most)
+constexpr uint32_t kIvRoot = 0x7FFFu, kIvStampShift = 18;
+constexpr uint32_t kIvMaxIters = 1u << 14;  // walk iterations the 14-bit stamps number
 
-// Appends a hit primitive child's crossings beyond 0; returns whether the
-// ray starts inside it.  `bad`: crossings other than (Enter, Exit) or (Exit);
-// `full`: more than kCoopMaxCross crossings.
-__device__ __forceinline__ bool IvCrossings(const SceneView& sv, const RayF& r, uint32_t desc, uint32_t owner,
-                                            float* ct, uint32_t* cb, uint32_t* co, uint32_t* count, bool& bad,
-                                            bool& full) {
-  Counts n{0, 0, 0, 0, 0, 0};
-  const Hit a = Intersect<true, false>(sv, r, desc, 0.0f, MissRecord(), n);
-  if (Flags(a.bits) & kMissBit) return false;
-  const bool aEnter = (a.bits & kEnter) != 0;
-  Hit b = MissRecord();
-  if (aEnter) {
-    b = Intersect<true, false>(sv, r, desc, a.t, MissRecord(), n);
-    if ((Flags(b.bits) & kMissBit) || !(b.bits & kExit)) bad = true;
-  } else if (!(a.bits & kExit)) {
-    bad = true;
+// A hit primitive's crossings beyond 0, as the reference's intersection
+// function reports them queried at tmin 0 and again at the first crossing
+// (the second query's value is the function's next root or cap beyond the
+// first).  Returns how many (0-2).
+__device__ __forceinline__ uint32_t PrimCrossings(const SceneView& sv, const RayF& r, uint32_t desc, Hit& a,
+                                                  Hit& b) {
+  const uint32_t type = DescType(desc), prim = DescIndex(desc);
+  const float4 pa = sv.primA[prim];
+  if (type == kSphereT) {
+    float t0, t1;
+    if (!SphereRoots<true>(r, pa, t0, t1)) return 0;
+    if (t0 > 0.0f) {
+      a = SphereAt(r, pa, prim, t0);
+      if (!(t1 > t0)) return 1;
+      b = SphereAt(r, pa, prim, t1);
+      return 2;
+    }
+    if (!(t1 > 0.0f)) return 0;
+    a = SphereAt(r, pa, prim, t1);
+    return 1;
   }
-  const uint32_t len = aEnter ? 2u : 1u;
-  const uint32_t at = atomicAdd(count, len);
-  if (at + len > kCoopMaxCross) {
-    full = true;
-    return false;
+  if (type == kCubeT) {
+    float tmn, tmx;
+    if (!CubeSlab<true>(r, pa, tmn, tmx)) return 0;
+    if (tmn > 0.0f) {
+      a = CubeAt(r, pa, prim, tmn);
+      if (!(tmx > tmn)) return 1;
+      b = CubeAt(r, pa, prim, tmx);
+      return 2;
+    }
+    if (!(tmx > 0.0f)) return 0;
+    a = CubeAt(r, pa, prim, tmx);
+    return 1;
   }
-  ct[at] = a.t;
-  cb[at] = a.bits;
-  co[at] = owner;
-  if (aEnter) {
-    ct[at + 1] = b.t;
-    cb[at + 1] = b.bits;
-    co[at + 1] = owner;
-  }
-  return !aEnter;
+  const float4 pb = sv.primB[prim];
+  a = CylinderHit<true>(r, pa, pb, prim, 0.0f, MissRecord());
+  if (Flags(a.bits) & kMissBit) return 0;
+  b = CylinderHit<true>(r, pa, pb, prim, a.t, MissRecord());
+  return (Flags(b.bits) & kMissBit) ? 1u : 2u;
 }
 
 // Returns false when the ray must go to the state machine instead.
 __device__ bool IntervalTrace(const SceneView& sv, const RayF& r, void* region, int32_t ops, uint16_t* q,
-                              uint32_t qmask, uint32_t* counts, int lane, Hit& res) {
+                              uint32_t qmask, int lane, Hit& res) {
   uint32_t* msk = reinterpret_cast<uint32_t*>(region);
-  uint32_t* meta = reinterpret_cast<uint32_t*>(msk + 2 * static_cast<size_t>(ops));
-  uint16_t* ready = reinterpret_cast<uint16_t*>(meta + ops);
+  uint32_t* meta = msk + 2 * static_cast<size_t>(ops);
+  uint16_t* entered = reinterpret_cast<uint16_t*>(meta + ops);
   float* ct = reinterpret_cast<float*>(static_cast<char*>(region) + CoopCrossOffset(ops));
   uint32_t* cb = reinterpret_cast<uint32_t*>(ct + kCoopMaxCross + 1);
   uint32_t* co = cb + kCoopMaxCross + 1;
-  uint32_t* nCross = counts;
-  uint32_t* nReady = counts + 1;
-  if (lane == 0) {
-    *nCross = 0u;
-    *nReady = 0u;
-  }
-  // A: the walk down, collecting crossings
+  uint2* jobs = reinterpret_cast<uint2*>(co + kCoopMaxCross + 1);  // kCoopJobs primitive tests
+  uint32_t nCross = 0, nEntered = 0;  // wave-uniform
+  // A: the walk down.  Each iteration enters the operators of the lanes'
+  // tasks (box tests, spawns), hands out the next iteration's tasks and
+  // starts loading their records, then tests the hit primitive children
+  // gathered from all lanes, one a lane, and appends their crossings.
   uint32_t mine = lane == 0 ? 0u : kNone;  // ENTER of the root, flip parity 0
   uint32_t head = 0, tail = 0;
-  WaveSync();
-  for (int iter = 0; iter < kIterCap; iter++) {
-    const uint64_t idle = __ballot(mine == kNone);
-    const uint32_t rank = LanesBelow(idle);
-    const uint32_t avail = tail - head, nIdle = static_cast<uint32_t>(__popcll(idle));
-    const uint32_t take = avail < nIdle ? avail : nIdle;
-    if (mine == kNone && rank < take) mine = q[(head + rank) & qmask];
-    head += take;
-    if (__ballot(mine != kNone) == 0) break;  // every hit operator entered
-    uint32_t sp0 = kNone, sp1 = kNone;
-    bool bad = false, full = false;
+  float4 q0 = make_float4(0, 0, 0, 0), q1 = q0, q2 = q0, q3 = q0;
+  if (mine != kNone) {
+    q0 = sv.ops[0];
+    q1 = sv.ops[1];
+    q2 = sv.ops[2];
+    q3 = sv.ops[3];
+  }
+  uint32_t iter = 0;
+  for (; iter < kIvMaxIters; iter++) {
+    uint32_t sp0 = kNone, sp1 = kNone, jobL = kNone, jobR = kNone, dL = 0, dR = 0;
     if (mine != kNone) {
       const uint32_t X = mine & 0x7FFFu, par = mine >> 15;
-      const float4 q0 = sv.ops[4 * X + 0];
-      const float4 q1 = sv.ops[4 * X + 1];
-      const float4 q2 = sv.ops[4 * X + 2];
-      const float4 q3 = sv.ops[4 * X + 3];
       const uint32_t lDesc = AsU(q3.x), rDesc = AsU(q3.y), typeWord = AsU(q3.z), parent = AsU(q3.w);
       const uint32_t op = typeWord & 0xFFu;
       const bool gL = BoxHit<true>(r, q0.x, q0.y, q0.z, q1.x, q1.y, q1.z, 0.0f);
       const bool gR = BoxHit<true>(r, q0.w, q1.w, q2.x, q2.y, q2.z, q2.w, 0.0f);
       const bool oL = gL & (DescType(lDesc) < kSphereT), oR = gR & (DescType(rDesc) < kSphereT);
       const uint32_t parR = par ^ (op == kOpDifference ? 1u : 0u);
-      bool inL = false, inR = false;
-      if (gL & !oL) inL = IvCrossings(sv, r, lDesc, (X << 2) | par, ct, cb, co, nCross, bad, full);
-      if (gR & !oR) inR = IvCrossings(sv, r, rDesc, (X << 2) | 2u | parR, ct, cb, co, nCross, bad, full);
-      if (!oL) msk[2 * X] = inL ? ~0u : 0u;
-      if (!oR) msk[2 * X + 1] = inR ? ~0u : 0u;
-      const uint32_t pend = static_cast<uint32_t>(oL) + static_cast<uint32_t>(oR);
+      if (gL & !oL) {
+        jobL = (X << 2) | par;
+        dL = lDesc;
+      } else if (!oL) {
+        msk[2 * X] = 0u;
+      }
+      if (gR & !oR) {
+        jobR = (X << 2) | 2u | parR;
+        dR = rDesc;
+      } else if (!oR) {
+        msk[2 * X + 1] = 0u;
+      }
       meta[X] = (parent == kNone ? kIvRoot : parent) | (op << 15) | (((typeWord >> 8) & 1u) << 17) |
-                (pend << kIvPendShift);
-      if (pend == 0) ready[atomicAdd(nReady, 1u)] = static_cast<uint16_t>(X);
+                (iter << kIvStampShift);
       if (oL) sp0 = DescIndex(lDesc) | (par << 15);
       if (oR) {
         if (oL) sp1 = DescIndex(rDesc) | (parR << 15);
         else sp0 = DescIndex(rDesc) | (parR << 15);
       }
     }
-    if (__ballot(bad | full) != 0) return false;
-    mine = sp0;
+    // the entered operators, in order (stamped with this iteration in meta)
+    const uint64_t be = __ballot(mine != kNone);
+    if (mine != kNone) entered[nEntered + LanesBelow(be)] = static_cast<uint16_t>(mine & 0x7FFFu);
+    nEntered += static_cast<uint32_t>(__popcll(be));
+    // queue the second spawns, gather the primitive tests
     const uint64_t bq = __ballot(sp1 != kNone);
     if (sp1 != kNone) q[(tail + LanesBelow(bq)) & qmask] = static_cast<uint16_t>(sp1);
     tail += static_cast<uint32_t>(__popcll(bq));
+    const uint64_t bl = __ballot(jobL != kNone), bR = __ballot(jobR != kNone);
+    const uint32_t nl = static_cast<uint32_t>(__popcll(bl));
+    const uint32_t nJobs = nl + static_cast<uint32_t>(__popcll(bR));
+    if (jobL != kNone) jobs[LanesBelow(bl)] = make_uint2(jobL, dL);
+    if (jobR != kNone) jobs[nl + LanesBelow(bR)] = make_uint2(jobR, dR);
     WaveSync();
+    // next iteration's tasks: the first spawn stays, idle lanes take queued
+    // ones in lane order; their records load during the primitive tests
+    mine = sp0;
+    const uint64_t idle = __ballot(mine == kNone);
+    const uint32_t rank = LanesBelow(idle);
+    const uint32_t avail = tail - head, nIdle = static_cast<uint32_t>(__popcll(idle));
+    const uint32_t take = avail < nIdle ? avail : nIdle;
+    if (mine == kNone && rank < take) mine = q[(head + rank) & qmask];
+    head += take;
+    if (mine != kNone) {
+      const uint32_t Y = mine & 0x7FFFu;
+      q0 = sv.ops[4 * Y + 0];
+      q1 = sv.ops[4 * Y + 1];
+      q2 = sv.ops[4 * Y + 2];
+      q3 = sv.ops[4 * Y + 3];
+    }
+    bool bad = false, full = false;
+    for (uint32_t base = 0; base < nJobs; base += 64) {
+      uint32_t c = 0, owner = 0;
+      Hit a = MissRecord(), b = MissRecord();
+      if (base + lane < nJobs) {
+        const uint2 jb = jobs[base + lane];
+        owner = jb.x;
+        c = PrimCrossings(sv, r, jb.y, a, b);
+        // (Enter, Exit) or (Exit) alone
+        if (c == 2) bad |= !(a.bits & kEnter) || !(b.bits & kExit);
+        else if (c == 1) bad |= !(a.bits & kExit);
+        msk[2 * (owner >> 2) + ((owner >> 1) & 1u)] = (c == 1) ? ~0u : 0u;  // inside from 0: an Exit alone
+      }
+      const uint64_t b1 = __ballot(c >= 1), b2 = __ballot(c == 2);
+      const uint32_t total = static_cast<uint32_t>(__popcll(b1) + __popcll(b2));
+      if (nCross + total > kCoopMaxCross) {
+        full = true;
+        break;
+      }
+      const uint32_t at = nCross + LanesBelow(b1) + LanesBelow(b2);
+      if (c >= 1) {
+        ct[at] = a.t;
+        cb[at] = a.bits;
+        co[at] = owner;
+      }
+      if (c == 2) {
+        ct[at + 1] = b.t;
+        cb[at + 1] = b.bits;
+        co[at + 1] = owner;
+      }
+      nCross += total;
+    }
+    if (full || __ballot(bad) != 0) return false;
+    WaveSync();
+    if (__ballot(mine != kNone) == 0) break;  // every hit operator entered
   }
-  WaveSync();
+  if (iter >= kIvMaxIters) return false;  // stamps would overflow (not reached by trees of 32767 operators)
   // B: sort the crossings, one a lane, and mark each primitive's intervals
-  const uint32_t k = __builtin_amdgcn_readfirstlane(*nCross);
+  const uint32_t k = nCross;
   float key = __builtin_inff();
   uint32_t bits = 0u, owner = 0u;
   if (static_cast<uint32_t>(lane) < k) {
@@ -1517,51 +1610,41 @@ __device__ bool IntervalTrace(const SceneView& sv, const RayF& r, void* region, 
   const float nextKey = __shfl_down(key, 1, 64);
   const bool tie = static_cast<uint32_t>(lane) + 1 < k && nextKey == key;
   if (__ballot(tie) != 0) return false;
-  const bool mine2 = static_cast<uint32_t>(lane) < k;
+  const bool held = static_cast<uint32_t>(lane) < k;
   const uint32_t slot = 2 * (owner >> 2) + ((owner >> 1) & 1u);
-  if (mine2 && (bits & kEnter)) atomicOr(&msk[slot], ~0u << (lane + 1));
+  if (held && (bits & kEnter)) atomicOr(&msk[slot], ~0u << (lane + 1));
   WaveSync();
-  if (mine2 && !(bits & kEnter)) atomicAnd(&msk[slot], (2u << lane) - 1u);
+  if (held && !(bits & kEnter)) atomicAnd(&msk[slot], (2u << lane) - 1u);
   WaveSync();
-  // C: combine masks bottom-up from the operators with no operator child hit
-  const uint32_t nr = __builtin_amdgcn_readfirstlane(*nReady);
-  uint32_t rhead = 0;
+  // C: combine masks bottom-up, the operators entered in one iteration at a
+  // time from the last iteration back (a child is always entered after its
+  // parent, so its mask is done when the parent's group runs)
   uint32_t rootMask = 0;
   bool done = false;
-  for (int iter = 0; iter < kIterCap; iter++) {
-    uint32_t task = kNone;
-    if (rhead + static_cast<uint32_t>(lane) < nr) task = ready[rhead + lane];
-    rhead += 64;
-    if (__ballot(task != kNone) == 0) break;  // every ready operator taken and no root: a malformed tree
-    bool fin = false;
-    uint32_t m = 0;
-    for (int pass = 0; pass < kIvPasses && task != kNone; pass++) {
-      const uint32_t md = meta[task];
-      const uint32_t mL = msk[2 * task], mR = msk[2 * task + 1];
+  for (uint32_t pos = nEntered; pos > 0;) {
+    const bool valid = static_cast<uint32_t>(lane) < pos;
+    const uint32_t X = valid ? entered[pos - 1 - lane] : 0u;
+    const uint32_t md = valid ? meta[X] : 0u;
+    const uint32_t s0 = __builtin_amdgcn_readfirstlane(md >> kIvStampShift);
+    const bool grp = valid && (md >> kIvStampShift) == s0;
+    if (grp) {
+      const uint32_t mL = msk[2 * X], mR = msk[2 * X + 1];
       const uint32_t op = (md >> 15) & 3u;
-      m = op == kOpUnion ? (mL | mR) : (op == kOpDifference ? (mL & ~mR) : (mL & mR));
+      const uint32_t m = op == kOpUnion ? (mL | mR) : (op == kOpDifference ? (mL & ~mR) : (mL & mR));
       const uint32_t parent = md & kIvRoot;
       if (parent == kIvRoot) {
-        fin = true;
-        task = kNone;
+        rootMask = m;
+        done = true;
       } else {
         msk[2 * parent + ((md >> 17) & 1u)] = m;
-        const uint32_t old = __hip_atomic_fetch_add(&meta[parent], 0u - kIvPendOne, __ATOMIC_ACQ_REL,
-                                                    __HIP_MEMORY_SCOPE_WAVEFRONT);
-        task = ((old >> kIvPendShift) & 3u) == 1u ? parent : kNone;
       }
     }
-    const uint64_t finished = __ballot(fin);
-    if (finished != 0) {
-      const int src = __ffsll(static_cast<unsigned long long>(finished)) - 1;
-      rootMask = __builtin_amdgcn_readlane(m, src);
-      done = true;
-      break;
-    }
+    pos -= static_cast<uint32_t>(__popcll(__ballot(grp)));
     WaveSync();
   }
-  WaveSync();
-  if (!done) return false;
+  const uint64_t rootLane = __ballot(done);
+  if (rootLane == 0) return false;  // no root among the entered: a malformed tree
+  rootMask = __builtin_amdgcn_readlane(rootMask, __ffsll(static_cast<unsigned long long>(rootLane)) - 1);
   // the first crossing at which the root's membership changes
   const uint32_t change = (rootMask ^ (rootMask >> 1)) & ((1u << k) - 1u);
   if (change == 0) {
@@ -1580,12 +1663,11 @@ __device__ bool IntervalTrace(const SceneView& sv, const RayF& r, void* region, 
 // operator states and the task ring, so a CU keeps up to 16 rays in flight
 // (the scene copy in LDS would leave room for fewer, which measured slower).
 template <bool kFast, bool kCount, bool kBatch>
-__global__ void __launch_bounds__(64) CoopKernel(RenderParams p) {
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) CoopKernel(RenderParams p) {
   extern __shared__ uint4 ldsBase[];
   __shared__ uint16_t table[kDecisionEntries];
   const int lane = threadIdx.x;
   if (lane < kDecisionEntries) table[lane] = kDecisions.v[lane];
-  __shared__ uint32_t ivCounts[2];
   // the state machine's view: operator states, the ring's tmins; the task
   // ring ends the region
   const uint32_t regionBytes = static_cast<uint32_t>(CoopWaveLds(p.coopOps, p.coopQueueCap, p.coopLists));
@@ -1657,7 +1739,7 @@ __global__ void __launch_bounds__(64) CoopKernel(RenderParams p) {
     Hit h = MissRecord();
     bool listed = false;
     if (kFast && !kCount && p.coopLists && jb.sc.conservative)
-      listed = IntervalTrace(sv, r, ldsBase, p.coopOps, q, qmask, ivCounts, lane, h);
+      listed = IntervalTrace(sv, r, ldsBase, p.coopOps, q, qmask, lane, h);
     if (listed) listedRays++;
     else h = CoopTrace<kFast, kCount>(sv, r, st, q, qt, qmask, table, lane, n, rec, ent, own);
     if (kCount && jb.counters) {

@@ -124,14 +124,16 @@ typedef struct csg_render_opts {
   uint32_t seed;        /* jitter seed */
   int32_t tile_walk;    /* 0: row-major tiles; 1: Morton order within a job (square power-of-two grids) */
   int32_t coop_budget;  /* state-machine steps a lane spends on one ray before the ray may be handed to a
-                           whole wave (cooperative evaluation, same results): 0 = default, < 0 = never */
+                           whole wave (cooperative evaluation, same results): 0 = default (384; 160 where
+                           the wave classifies by intervals, see coop_intervals), < 0 = never */
   int32_t coop_when;    /* 0: hand over only in the frame's tail, once every tile has been taken and a
                            further coop_grace percent of the time that took has passed;
                            1: as soon as the budget is spent */
   int32_t coop_grace;   /* 0 = default: 950 x waves / tiles percent (at most 300) for one scene, 550 x for
-                           a batch (the tiles in flight when the queue runs dry finish in about a tile's
-                           time; handing out took tiles / waves tile times): 120 for a 1024-lane launch of
-                           1920x1080, about 3 for the 256-tree batch; < 0 = none; else percent */
+                           a batch or where the wave classifies by intervals (the tiles in flight when the
+                           queue runs dry finish in about a tile's time; handing out took tiles / waves
+                           tile times): 120 (70 with intervals) for a 1024-lane launch of 1920x1080, about
+                           3 for the 256-tree batch; < 0 = none; else percent */
   int32_t coop_intervals; /* 0 = default: in fast mode, a handed-over ray of a scene whose primitives all
                              lie inside their boxes is classified by intervals (one pass over the hit part
                              of the tree, the crossings sorted, membership masks combined bottom-up; the

@@ -68,7 +68,7 @@ def test_cheese_split_between_kernels_matches_oracle(gpu):
     o = _oracle_exact(text, cam6, l3, ph8, 480, 270)
     for budget, when in ((0, 0), (64, 0), (64, 1)):
         g = _np(s.render(cam, light, phong, 480, 270, mode=C.MODE_EXACT, counters=True, coop=budget, coop_when=when))
-        heavy = int(((g["counters"][..., 0] + g["counters"][..., 1]) > (budget or s.plan().coop_budget)).sum())
+        heavy = int(((g["counters"][..., 0] + g["counters"][..., 1]) > (budget or s.plan(mode=C.MODE_EXACT).coop_budget)).sum())
         print(f"budget={budget} rays over budget={heavy}")
         assert heavy > 0
         assert np.array_equal(g["prim_id"], o["prim_id"])
