@@ -872,11 +872,14 @@ __device__ __forceinline__ void RenderTile(const Job& jb, const SceneView& sv, u
   if (exceeded) {
     // Hand the ray to the cooperative kernel; if its list is full, finish
     // here.  The first ResumeCap rays leave their state, so the cooperative
-    // kernel continues them where this kernel stopped (the others restart).
+    // kernel continues them where this kernel stopped (the others restart),
+    // unless the cooperative kernel will classify them by intervals (which
+    // starts from the ray's origin).
     const uint32_t slot = atomicAdd(p.queue + 2, 1u);
     if (slot < p.heavyCap) {
       const uint32_t rcap = ResumeCap(p, cap);
-      const bool keep = slot < rcap && ts.slots + 2 <= 64 && ts.cur != kNone;
+      const bool classify = kFast && !kCount && p.coopLists && jb.sc.conservative;
+      const bool keep = !classify && slot < rcap && ts.slots + 2 <= 64 && ts.cur != kNone;
       p.heavy[slot] = make_uint2(jobIndex | (keep ? kResumeFlag : 0u), static_cast<uint32_t>(out));
       if (keep) {
         uint32_t* rec = p.resume + static_cast<size_t>(slot) * kResumeWords;
