@@ -145,6 +145,24 @@ def test_interval_classification_on_heavy_scenes(gpu, make):
     assert handed > 0 and classified > 0.9 * handed
 
 
+@pytest.mark.parametrize("eye,target", [((0.05, 0.02, 0.03), (1.0, 0.4, 0.2)), ((0.9, -0.7, 0.2), (-1.0, 0.3, -0.4))],
+                         ids=["centre", "near_corner"])
+def test_interval_classification_from_inside_the_solid(gpu, eye, target):
+    """The camera inside the cheese cube: every ray starts inside the cube
+    (its first crossing a lone Exit) and some inside spheres; classification,
+    state machine and per-lane kernel agree bit for bit."""
+    print(f"\n[case] cheese 512 from inside, eye={eye}, 480x270 fast", flush=True)
+    light, phong = C.light(S.LIGHT_TOWARD), C.phong(*S.PHONG)
+    s = C.Scene(S.cheese(512, seed=512))
+    cam = C.camera_look_at(eye, target, 70)
+    lane, iv, machine, classified, handed = _three_ways(s, cam, light, phong, 480, 270)
+    s.close()
+    print(f"handed={handed} classified={classified} hits={int((lane['prim_id'] >= 0).sum())}")
+    _same(iv, lane)
+    _same(machine, lane)
+    assert handed > 0 and classified > 0.9 * handed
+
+
 def test_heavy_list_overflow_finishes_per_lane(gpu):
     """2100x2100 rays with budget 1 overflow the 4 Mi-entry heavy list; the
     rays past its end finish in the per-lane kernel with the same results."""
