@@ -19,7 +19,8 @@ struct DevScene {
   int32_t nPrims;
   int32_t opDepth;
   int32_t opLeaves;  // operators without an operator child: a ray's live cooperative tasks never exceed it
-  uint32_t conservative;  // 1: every primitive lies inside its box (the cooperative list evaluation applies)
+  uint32_t conservative;  // 1: every primitive lies inside its box (interval classification applies)
+  const float4* ivOps;    // conservative scenes: operator records with boxes tight around each child's solid
 };
 
 // One render job: a scene seen by one camera at one resolution, with its

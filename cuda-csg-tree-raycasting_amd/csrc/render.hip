@@ -1741,8 +1741,11 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) Co
     }
     Hit h = MissRecord();
     bool listed = false;
-    if (kFast && !kCount && p.coopLists && jb.sc.conservative)
-      listed = IntervalTrace(sv, r, ldsBase, p.coopOps, q, qmask, lane, h);
+    if (kFast && !kCount && p.coopLists && jb.sc.conservative) {
+      // the walk culls with boxes tight around each subtree's solid
+      const SceneView tv{jb.sc.ivOps ? jb.sc.ivOps : jb.sc.ops, jb.sc.primA, jb.sc.primB, jb.sc.ops, 0u};
+      listed = IntervalTrace(tv, r, ldsBase, p.coopOps, q, qmask, lane, h);
+    }
     if (listed) listedRays++;
     else h = CoopTrace<kFast, kCount>(sv, r, st, q, qt, qmask, table, lane, n, rec, ent, own);
     if (kCount && jb.counters) {

@@ -303,6 +303,66 @@ FlatScene Flatten(const CSGTree& tree) {
     f.primC[p] = F4{q.r, q.g, q.b, 0.0f};
     if (q.type == kCylinder && !CylinderInBox(q)) f.conservative = false;
   }
+  if (f.conservative && f.nOps > 0) {
+    // Boxes around each node's solid, for culling in interval
+    // classification (any box that contains a subtree's solid gives the same
+    // crossings there, as the solid has none outside it): a primitive's own
+    // extent (a sphere's c +- r, not the reference's 2r; a cylinder's from
+    // its axis) widened by 1e-4 of its size, so that a ray that meets the
+    // solid crosses the box well clear of the slab test's rounding; a Union
+    // the union of its children's, a Difference its left child's, an
+    // Intersection the overlap of both (lo > hi where they do not overlap: no
+    // ray hits it).
+    std::vector<Box> tight(tree.nodes.size());
+    for (size_t k = tree.nodes.size(); k-- > 0;) {
+      const CSGNode& n = tree.nodes[k];
+      Box& b = tight[k];
+      if (n.primitiveIdx != -1) {
+        const Primitive& q = tree.primitives[n.primitiveIdx];
+        const double c[3] = {q.x, q.y, q.z};
+        for (int a = 0; a < 3; a++) {
+          double e;
+          if (q.type == kSphere) {
+            e = q.radius;
+          } else if (q.type == kCube) {
+            e = q.size / 2.0;
+          } else {
+            const double v = q.axis[a];
+            e = std::fabs(v) * (q.height / 2.0) + q.radius * std::sqrt(std::max(0.0, 1.0 - v * v));
+          }
+          const double pad = 1e-4 * e + 1e-6 * std::fabs(c[a]);
+          b.lo[a] = static_cast<float>(c[a] - e - pad);
+          b.hi[a] = static_cast<float>(c[a] + e + pad);
+        }
+      } else {
+        const Box& L = tight[n.left];
+        const Box& R = tight[n.right];
+        for (int a = 0; a < 3; a++) {
+          if (n.type == kUnion) {
+            b.lo[a] = std::min(L.lo[a], R.lo[a]);
+            b.hi[a] = std::max(L.hi[a], R.hi[a]);
+          } else if (n.type == kDifference) {
+            b.lo[a] = L.lo[a];
+            b.hi[a] = L.hi[a];
+          } else {
+            b.lo[a] = std::max(L.lo[a], R.lo[a]);
+            b.hi[a] = std::min(L.hi[a], R.hi[a]);
+          }
+        }
+      }
+    }
+    f.ivOps = f.ops;
+    for (int32_t i = 0; i < f.nNodes; i++) {
+      const CSGNode& n = tree.nodes[i];
+      if (n.primitiveIdx != -1) continue;
+      const Box& L = tight[n.left];
+      const Box& R = tight[n.right];
+      F4* q = &f.ivOps[4 * static_cast<size_t>(opIndex[i])];
+      q[0] = F4{L.lo[0], L.lo[1], L.lo[2], R.lo[0]};
+      q[1] = F4{L.hi[0], L.hi[1], L.hi[2], R.lo[1]};
+      q[2] = F4{R.lo[2], R.hi[0], R.hi[1], R.hi[2]};
+    }
+  }
   return f;
 }
 

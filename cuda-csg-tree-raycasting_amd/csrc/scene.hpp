@@ -86,6 +86,10 @@ struct alignas(16) F4 {
 
 struct FlatScene {
   std::vector<F4> ops, primA, primB, primC;
+  // For scenes whose primitives all lie in their boxes: the operator records
+  // again with each child's box tight around the child's solid (below), for
+  // the cooperative kernel's interval classification; empty otherwise.
+  std::vector<F4> ivOps;
   uint32_t rootDesc = 0;
   int32_t nNodes = 0, nOps = 0, nPrims = 0, opDepth = 0;
   int32_t opLeaves = 0;  // operators without an operator child
