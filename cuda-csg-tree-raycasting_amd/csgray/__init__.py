@@ -57,7 +57,7 @@ class RenderOpts(ctypes.Structure):
                 ("counters", ctypes.c_int32), ("counter_out", ctypes.c_void_p), ("tile_order", ctypes.c_void_p),
                 ("samples", ctypes.c_int32), ("seed", ctypes.c_uint32), ("tile_walk", ctypes.c_int32),
                 ("coop_budget", ctypes.c_int32), ("coop_when", ctypes.c_int32), ("coop_grace", ctypes.c_int32),
-                ("coop_intervals", ctypes.c_int32)]
+                ("coop_intervals", ctypes.c_int32), ("tight_boxes", ctypes.c_int32)]
 
 
 class Job(ctypes.Structure):
@@ -295,7 +295,7 @@ class Scene:
 
     def render(self, cam, light_, phong_, width, height, mode=MODE_FAST, outputs=None, counters=False,
                stream=None, device=None, block=0, placement=0, tile_order=None, samples=1, seed=0, walk=0, coop=0,
-               coop_when=0, coop_grace=0, coop_intervals=0):
+               coop_when=0, coop_grace=0, coop_intervals=0, tight_boxes=0):
         """Render into torch CUDA tensors; returns dict rgba/depth/prim_id/flags(/counters).
 
         `outputs` may pass preallocated tensors (same keys) to avoid allocation.  `coop` is the
@@ -303,7 +303,8 @@ class Scene:
         `coop_when` 0 moves it only in the frame's tail (once every tile has been taken and a further
         `coop_grace` percent of the time that took has passed; 0 default, < 0 none), 1 as soon as the
         budget is spent.  `coop_intervals` < 0 keeps handed-over rays on the state machine (0: interval
-        classification where the scene allows, fast mode)."""
+        classification where the scene allows, fast mode); `tight_boxes` < 0 culls with the reference's
+        boxes in fast mode too."""
         import torch
         dev = device or torch.device("cuda", torch.cuda.current_device())
         spp = max(1, int(samples))
@@ -318,7 +319,7 @@ class Scene:
         opts = RenderOpts(mode, block, placement, 1 if counters else 0,
                           outputs["counters"].data_ptr() if counters else None,
                           tile_order.data_ptr() if tile_order is not None else None, spp, seed, walk, coop,
-                          coop_when, coop_grace, coop_intervals)
+                          coop_when, coop_grace, coop_intervals, tight_boxes)
         s = stream if stream is not None else torch.cuda.current_stream(dev).cuda_stream
         _check(lib().csg_render_ex(self.handle, ctypes.byref(cam), ctypes.byref(light_), ctypes.byref(phong_),
                                    int(width), int(height), ctypes.byref(o), ctypes.byref(opts), s))
@@ -430,11 +431,11 @@ def batch_order(jobs):
 
 
 def render_batch(jobs, mode=MODE_FAST, block=0, walk=0, samples=1, seed=0, stream=None, coop=0, coop_when=0,
-                 coop_grace=0, coop_intervals=0):
+                 coop_grace=0, coop_intervals=0, tight_boxes=0):
     """One persistent launch over a ctypes Job array (see make_jobs)."""
     import torch
     opts = RenderOpts(mode, block, PLACE_GLOBAL, 0, None, None, samples, seed, walk, coop, coop_when, coop_grace,
-                      coop_intervals)
+                      coop_intervals, tight_boxes)
     s = stream if stream is not None else torch.cuda.current_stream().cuda_stream
     _check(lib().csg_render_batch(jobs, len(jobs), ctypes.byref(opts), s))
 
@@ -443,9 +444,9 @@ class Batch:
     """A captured batch (csg_batch*): jobs fixed at creation, graph replayed by launch()."""
 
     def __init__(self, jobs, mode=MODE_FAST, block=0, walk=0, samples=1, seed=0, coop=0, coop_when=0, coop_grace=0,
-                 coop_intervals=0):
+                 coop_intervals=0, tight_boxes=0):
         opts = RenderOpts(mode, block, PLACE_GLOBAL, 0, None, None, samples, seed, walk, coop, coop_when, coop_grace,
-                          coop_intervals)
+                          coop_intervals, tight_boxes)
         h = _P()
         _check(lib().csg_batch_create(jobs, len(jobs), ctypes.byref(opts), ctypes.byref(h)))
         self.handle = h

@@ -441,6 +441,9 @@ csg_status BuildJob(const csg_scene* s, const csg_camera* cam, const csg_light* 
   if (dev != s->device) return Fail(CSG_E_ARG, "csg_render: scene was uploaded to another device");
   std::memset(jb, 0, sizeof(*jb));
   jb->sc = s->dev;
+  // fast mode culls conservative scenes with their tight boxes
+  const bool fast = !opts || opts->mode == CSG_MODE_FAST;
+  if (fast && (!opts || opts->tight_boxes >= 0) && s->dev.ivOps) jb->sc.ops = s->dev.ivOps;
   const csgray::CameraBasis b = csgray::MakeBasis(cam->pos, cam->pitch, cam->yaw, cam->vfov);
   for (int k = 0; k < 3; k++) {
     jb->pos[k] = b.pos[k];

@@ -239,6 +239,38 @@ def test_fast_mode_culls_like_the_reference_on_axis_parallel_rays(gpu):
         assert np.array_equal(g["prim_id"][:, col], ref["prim_id"][:, col]), mode
 
 
+@pytest.mark.parametrize("setup", [0, 1])
+@pytest.mark.parametrize("name", scene_names() + ["cheese512"])
+def test_fast_tight_boxes_change_no_unflagged_ray(gpu, name, setup):
+    """Fast mode culls scenes whose primitives lie in their boxes with boxes
+    tight around each subtree's solid: every ray the FP64 oracle does not
+    flag gets the same hit id, flags and depth bits as with the reference's
+    boxes, and the per-lane kernel runs fewer steps."""
+    from csgray import scenes as S
+    sname, cam, light, phong, cam6, l3, ph8 = setups()[setup]
+    text = S.cheese(512, seed=512) if name == "cheese512" else scene_text(name)
+    w, h = (480, 270) if name == "cheese512" else (160, 90)
+    print(f"\n[case] {name} setup={setup} tight vs reference boxes", flush=True)
+    s = C.Scene(text)
+    inside = s.info().prims_in_boxes
+    t = {k: v.cpu().numpy() for k, v in s.render(cam, light, phong, w, h, counters=True, coop=-1).items()}
+    r = {k: v.cpu().numpy() for k, v in s.render(cam, light, phong, w, h, counters=True, coop=-1,
+                                                 tight_boxes=-1).items()}
+    s.close()
+    o = O.OracleScene(text).render(cam6, l3, ph8, w, h, bits=64, degen=True)
+    ok = o["degen"] == 0
+    for k in ("prim_id", "flags"):
+        assert np.array_equal(t[k][ok], r[k][ok]), k
+    assert np.array_equal(t["depth"][ok].view(np.uint32), r["depth"][ok].view(np.uint32))
+    steps_t = (t["counters"][..., 0] + t["counters"][..., 1]).sum()
+    steps_r = (r["counters"][..., 0] + r["counters"][..., 1]).sum()
+    print(f"prims_in_boxes={inside} steps tight/reference={steps_t}/{steps_r} flagged={1 - ok.mean():.4%}")
+    if inside:
+        assert steps_t <= steps_r
+    else:
+        assert steps_t == steps_r
+
+
 def test_launch_plan(gpu):
     s = C.Scene(scene_text("testCheese512.txt"))
     auto = s.plan()
