@@ -20,7 +20,7 @@ struct DevScene {
   int32_t opDepth;
   int32_t opLeaves;  // operators without an operator child: a ray's live cooperative tasks never exceed it
   uint32_t conservative;  // 1: every primitive lies inside its box (interval classification applies)
-  const float4* ivOps;    // conservative scenes: operator records with boxes tight around each child's solid
+  const float4* ivOps;    // operator records with fast mode's culling boxes (tight where the reference allows)
 };
 
 // One render job: a scene seen by one camera at one resolution, with its
@@ -193,8 +193,10 @@ constexpr size_t kCoopOpStateBytes = 16;
 constexpr size_t kCoopTaskBytes = 6;
 constexpr uint32_t kCoopMaxCross = 31;  // crossings a ray's interval masks (32 bits) can order
 constexpr size_t kCoopJobs = 128;       // primitive tests one iteration gathers (two a lane)
+// Interval classification's crossings start after the per-operator state
+// (14 B an operator), 8-byte aligned for the primitive tests that follow them.
 CSG_HOST_DEVICE inline size_t CoopCrossOffset(int32_t ops) {
-  return (static_cast<size_t>(ops) * 14 + 3) & ~static_cast<size_t>(3);
+  return (static_cast<size_t>(ops) * 14 + 7) & ~static_cast<size_t>(7);
 }
 CSG_HOST_DEVICE inline size_t CoopWaveLds(int32_t ops, uint32_t queueCap, int32_t lists) {
   const size_t machine =

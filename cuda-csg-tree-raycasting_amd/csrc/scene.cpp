@@ -303,22 +303,33 @@ FlatScene Flatten(const CSGTree& tree) {
     f.primC[p] = F4{q.r, q.g, q.b, 0.0f};
     if (q.type == kCylinder && !CylinderInBox(q)) f.conservative = false;
   }
-  if (f.conservative && f.nOps > 0) {
-    // Boxes around each node's solid, for culling in interval
-    // classification (any box that contains a subtree's solid gives the same
-    // crossings there, as the solid has none outside it): a primitive's own
-    // extent (a sphere's c +- r, not the reference's 2r; a cylinder's from
-    // its axis) widened by 1e-4 of its size, so that a ray that meets the
-    // solid crosses the box well clear of the slab test's rounding; a Union
-    // the union of its children's, a Difference its left child's, an
-    // Intersection the overlap of both (lo > hi where they do not overlap: no
-    // ray hits it).
+  if (f.nOps > 0) {
+    // Culling boxes for fast mode, each containing what the reference's
+    // culling lets a ray see of its subtree, so that they cull the same rays
+    // and fewer of the rays that meet nothing: a primitive that lies inside
+    // its reference box gets its own extent (a sphere's c +- r, not the
+    // reference's 2r; a cylinder's from its axis) widened by 1e-4 of its
+    // size, so that a ray that meets the solid crosses the box well clear of
+    // the slab test's rounding; a tilted cylinder that pokes out of its
+    // reference box keeps that box (the reference's culling decides which of
+    // its crossings a ray sees).  A Union takes the union of its children's
+    // boxes; a Difference its left child's (it is empty wherever the left
+    // operand is); an Intersection, when both operands lie inside their
+    // boxes, the overlap of both (lo > hi where they do not overlap: no ray
+    // hits it), else the union.  Interval classification culls with the
+    // same boxes.
     std::vector<Box> tight(tree.nodes.size());
+    std::vector<uint8_t> inside(tree.nodes.size(), 1);
     for (size_t k = tree.nodes.size(); k-- > 0;) {
       const CSGNode& n = tree.nodes[k];
       Box& b = tight[k];
       if (n.primitiveIdx != -1) {
         const Primitive& q = tree.primitives[n.primitiveIdx];
+        if (q.type == kCylinder && !CylinderInBox(q)) {
+          b = LeafBox(q);
+          inside[k] = 0;
+          continue;
+        }
         const double c[3] = {q.x, q.y, q.z};
         for (int a = 0; a < 3; a++) {
           double e;
@@ -337,16 +348,18 @@ FlatScene Flatten(const CSGTree& tree) {
       } else {
         const Box& L = tight[n.left];
         const Box& R = tight[n.right];
+        inside[k] = inside[n.left] & inside[n.right];
+        const bool overlap = n.type == kIntersection && inside[k];
         for (int a = 0; a < 3; a++) {
-          if (n.type == kUnion) {
-            b.lo[a] = std::min(L.lo[a], R.lo[a]);
-            b.hi[a] = std::max(L.hi[a], R.hi[a]);
-          } else if (n.type == kDifference) {
+          if (n.type == kDifference) {
             b.lo[a] = L.lo[a];
             b.hi[a] = L.hi[a];
-          } else {
+          } else if (overlap) {
             b.lo[a] = std::max(L.lo[a], R.lo[a]);
             b.hi[a] = std::min(L.hi[a], R.hi[a]);
+          } else {
+            b.lo[a] = std::min(L.lo[a], R.lo[a]);
+            b.hi[a] = std::max(L.hi[a], R.hi[a]);
           }
         }
       }

@@ -86,9 +86,9 @@ struct alignas(16) F4 {
 
 struct FlatScene {
   std::vector<F4> ops, primA, primB, primC;
-  // For scenes whose primitives all lie in their boxes: the operator records
-  // again with each child's box tight around the child's solid (below), for
-  // the cooperative kernel's interval classification; empty otherwise.
+  // The operator records again with fast mode's culling boxes (tight around
+  // each child's solid where the reference's culling allows; Flatten); empty
+  // for a scene without operators.
   std::vector<F4> ivOps;
   uint32_t rootDesc = 0;
   int32_t nNodes = 0, nOps = 0, nPrims = 0, opDepth = 0;

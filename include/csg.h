@@ -139,9 +139,10 @@ typedef struct csg_render_opts {
                              of the tree, the crossings sorted, membership masks combined bottom-up; the
                              state machine's result, which a ray gets whenever the two could order events
                              differently); < 0 = state machine only */
-  int32_t tight_boxes;  /* 0 = default: in fast mode, a scene whose primitives all lie inside their boxes
-                           is culled with boxes tight around each subtree's solid (a sphere's c +- r where
-                           the reference builds c +- 2r; a Difference's left operand's box; an
+  int32_t tight_boxes;  /* 0 = default: fast mode culls with boxes tight around each subtree's solid
+                           where the reference's culling allows (a sphere's c +- r where the reference
+                           builds c +- 2r; a cylinder's own extent unless it pokes out of the
+                           reference's box, which it then keeps; a Difference's left operand's box; an
                            Intersection's overlap): the same results on every ray whose culling the
                            reference does not decide within a rounding margin, in fewer steps;
                            < 0 = the reference's boxes */
