@@ -1694,11 +1694,18 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) Co
   uint32_t* heavyCtr = p.queue + kHeavyStripeWord;
   uint32_t listedRays = 0;  // rays this wave finished by interval classification
   uint32_t stripe = blockIdx.x % kQueueStripes;  // home stripe; then any stripe with rays left
+  // The next slot of the wave's stripe is claimed while the current ray is
+  // traced (its atomic's latency hidden), `ahead` until it is used.
+  uint32_t ahead = kNone;
   for (;;) {
     __builtin_amdgcn_wave_barrier();
-    uint32_t j = 0;
-    if (lane == 0) j = atomicAdd(heavyCtr + stripe * kStripeWords, 1u);
-    j = __builtin_amdgcn_readfirstlane(j);
+    uint32_t j = ahead;
+    if (ahead == kNone) {
+      j = 0;
+      if (lane == 0) j = atomicAdd(heavyCtr + stripe * kStripeWords, 1u);
+      j = __builtin_amdgcn_readfirstlane(j);
+    }
+    ahead = kNone;
     const uint32_t k = stripe + kQueueStripes * j;
     if (k >= count) {
       const uint32_t c = lane < static_cast<int>(kQueueStripes)
@@ -1712,6 +1719,8 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) Co
       stripe = NextStripe(m, stripe);
       continue;
     }
+    uint32_t next = 0;
+    if (lane == 0) next = atomicAdd(heavyCtr + stripe * kStripeWords, 1u);
     const uint2 e = p.heavy[k];
     const uint32_t jobWord = __builtin_amdgcn_readfirstlane(e.x);
     const Job& jb = kBatch ? p.jobs[jobWord & ~kResumeFlag] : p.job;
@@ -1757,6 +1766,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) Co
       for (int i = 0; i < 6; i++) c[i] = v[i];
     }
     ShadeAndStore<kFast>(jb, sv, r, h, out);
+    ahead = __builtin_amdgcn_readfirstlane(next);
   }
   // The last wave out adds the launch's heavy rays to the stream's running
   // total (words 6-7) and returns the heavy-ray counters to zero; each wave

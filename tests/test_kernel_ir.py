@@ -7,7 +7,9 @@ branch on the lane index at the end of the loop body into the lane-0 fetch
 at its head, so that lanes 1-63 loop back to the readfirstlane without lane 0
 and read a stale index forever.  The IR must show, for every kernel
 instantiation, that the broadcast value is a phi of exactly the atomic's
-result and the constant 0 of the lanes that skipped it.
+result and the constant 0 of the lanes that skipped it.  (The cooperative
+kernel has two such pops: the fetch of a ray when none is claimed, and the
+claim of the next one while the current ray is traced.)
 """
 import os
 import re
@@ -55,4 +57,4 @@ def test_queue_pop_broadcast_is_not_threaded(device_ir):
                 continue
             pops += 1
             assert len(ins) == 2 and "0" in [v for v, _ in ins], (name, d)
-        assert pops == 1, name
+        assert pops == (2 if "CoopKernel" in name else 1), name
