@@ -166,15 +166,21 @@ CoopPlan PlanCoop(const LaunchConfig& cfg, int32_t maxOps, int32_t maxLeaves, bo
 // tail (the last tile goes out after one or two tile times), so a single
 // scene's grace stays at or below 300 %: the 256 random trees of the
 // batched mix, each in its own launch, 222 -> 215.5 ms, the cheese frames
-// unchanged (a cap of 150 % would cost its 1024x1024 frame 10 %).  Where
-// the cooperative kernel classifies rays by intervals (`lists`), a handed-
-// over ray costs it about half as much, and the tail is best handed over
-// sooner: k = 5500 for single scenes too (cheese frames, measured best:
-// 60-80 % at 1920x1080, 20 % or less at 3840x2160, 100 % or more at
-// 1024x1024; k = 5500 gives 70, 17 and 137 %).
+// unchanged (a cap of 150 % would cost its 1024x1024 frame 10 %).
+// Where the cooperative kernel classifies rays by intervals (`lists`) a
+// handed-over ray costs it less, and the best grace falls off faster than
+// waves / tiles as frames grow (cheese, medians of five interleaved runs:
+// 1280x720 150-200 %, 1024x1024 200-250 %, 1920x1080 88-90 %, 2560x1440
+// 25-38 %, 3840x2160 7-11 %): 2000 % x (waves / tiles)^1.5, at most 200 %
+// (200, 200, 90, 38 and 11 % for those frames).
 int32_t DefaultCoopGraceMille(int64_t waves, int64_t tiles, bool batched, bool lists) {
   if (tiles <= 0) return 1000;
-  const int64_t k = (batched || lists) ? 5500 : 9500;
+  if (lists && !batched) {
+    const double r = static_cast<double>(waves) / static_cast<double>(tiles);
+    const double g = 20000.0 * r * std::sqrt(r);
+    return static_cast<int32_t>(std::min(2000.0, std::max(5.0, g + 0.5)));
+  }
+  const int64_t k = batched ? 5500 : 9500;
   const int64_t g = (k * waves + tiles / 2) / tiles;
   return static_cast<int32_t>(std::min<int64_t>(std::max<int64_t>(g, 5), batched ? 1000000 : 3000));
 }

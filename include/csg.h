@@ -130,9 +130,10 @@ typedef struct csg_render_opts {
                            further coop_grace percent of the time that took has passed;
                            1: as soon as the budget is spent */
   int32_t coop_grace;   /* 0 = default: 950 x waves / tiles percent (at most 300) for one scene, 550 x for
-                           a batch or where the wave classifies by intervals (the tiles in flight when the
-                           queue runs dry finish in about a tile's time; handing out took tiles / waves
-                           tile times): 120 (70 with intervals) for a 1024-lane launch of 1920x1080, about
+                           a batch (the tiles in flight when the queue runs dry finish in about a tile's
+                           time; handing out took tiles / waves tile times), and 2000 x (waves /
+                           tiles)^1.5 (at most 200) for one scene whose rays the wave classifies by
+                           intervals: 120 (90 with intervals) for a 1024-lane launch of 1920x1080, about
                            3 for the 256-tree batch; < 0 = none; else percent */
   int32_t coop_intervals; /* 0 = default: in fast mode, a handed-over ray of a scene whose primitives all
                              lie inside their boxes is classified by intervals (one pass over the hit part
