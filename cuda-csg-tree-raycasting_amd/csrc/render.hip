@@ -1387,7 +1387,7 @@ __device__ Hit CoopTrace(const SceneView& sv, const RayF& r, OpState* st, uint16
 //      hit primitive child: the reference's own intersection functions at
 //      tmin 0 and again at the first crossing, so times and hit bits are the
 //      state machine's;
-//   B. the wave sorts the k <= 31 crossings by time (bitonic, one per lane);
+//   B. the wave sorts the k <= 31 crossings by time (one per lane, each placed by a count);
 //      interval i of the ray is the stretch before crossing i (i = k: after
 //      the last), and each primitive's membership over the intervals becomes
 //      a 32-bit mask (bits after its Enter, up to its Exit);
@@ -1589,7 +1589,11 @@ __device__ bool IntervalTrace(const SceneView& sv, const RayF& r, void* region, 
     if (__ballot(mine != kNone) == 0) break;  // every hit operator entered
   }
   if (iter >= kIvMaxIters) return false;  // stamps would overflow (not reached by trees of 32767 operators)
-  // B: sort the crossings, one a lane, and mark each primitive's intervals
+  // B: sort the crossings, one a lane, and mark each primitive's intervals.
+  // A lane's place is the number of crossings before its own in (time,
+  // index) order, counted against each crossing broadcast in turn (k <= 31
+  // scalar reads, no LDS round trips); any two of equal time send the ray
+  // to the state machine.
   const uint32_t k = nCross;
   float key = __builtin_inff();
   uint32_t bits = 0u, owner = 0u;
@@ -1598,21 +1602,25 @@ __device__ bool IntervalTrace(const SceneView& sv, const RayF& r, void* region, 
     bits = cb[lane];
     owner = co[lane];
   }
-  for (int size = 2; size <= 32; size <<= 1) {  // k < 32: lanes 32-63 hold +inf
-    for (int stride = size >> 1; stride > 0; stride >>= 1) {
-      const float ok = __shfl_xor(key, stride, 64);
-      const uint32_t ob = __shfl_xor(bits, stride, 64), oo = __shfl_xor(owner, stride, 64);
-      const bool up = (lane & size) == 0, low = (lane & stride) == 0;
-      if (low == up ? ok < key : ok > key) {
-        key = ok;
-        bits = ob;
-        owner = oo;
-      }
-    }
+  uint32_t place = 0;
+  bool tie = false;
+  for (uint32_t i = 0; i < k; i++) {
+    const float ki = __uint_as_float(__builtin_amdgcn_readlane(AsU(key), static_cast<int>(i)));
+    place += (ki < key || (ki == key && i < static_cast<uint32_t>(lane))) ? 1u : 0u;
+    tie |= ki == key && i != static_cast<uint32_t>(lane);
   }
-  const float nextKey = __shfl_down(key, 1, 64);
-  const bool tie = static_cast<uint32_t>(lane) + 1 < k && nextKey == key;
-  if (__ballot(tie) != 0) return false;
+  if (__ballot(tie && static_cast<uint32_t>(lane) < k) != 0) return false;
+  if (static_cast<uint32_t>(lane) < k) {
+    ct[place] = key;
+    cb[place] = bits;
+    co[place] = owner;
+  }
+  WaveSync();
+  if (static_cast<uint32_t>(lane) < k) {
+    key = ct[lane];
+    bits = cb[lane];
+    owner = co[lane];
+  }
   const bool held = static_cast<uint32_t>(lane) < k;
   const uint32_t slot = 2 * (owner >> 2) + ((owner >> 1) & 1u);
   if (held && (bits & kEnter)) atomicOr(&msk[slot], ~0u << (lane + 1));
