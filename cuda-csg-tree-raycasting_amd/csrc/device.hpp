@@ -192,7 +192,7 @@ size_t CoopLdsBytes(int32_t ops, int32_t leaves, bool lists, uint32_t* queueCap)
 constexpr size_t kCoopOpStateBytes = 16;
 constexpr size_t kCoopTaskBytes = 6;
 constexpr uint32_t kCoopMaxCross = 31;  // crossings a ray's interval masks (32 bits) can order
-constexpr size_t kCoopJobs = 128;       // primitive tests one iteration gathers (two a lane)
+constexpr size_t kCoopJobs = 192;       // pending primitive tests: fewer than 64 left over + two a lane
 // Interval classification's crossings start after the per-operator state
 // (14 B an operator), 8-byte aligned for the primitive tests that follow them.
 CSG_HOST_DEVICE inline size_t CoopCrossOffset(int32_t ops) {

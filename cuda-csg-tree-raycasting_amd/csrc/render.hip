@@ -1477,11 +1477,13 @@ __device__ bool IntervalTrace(const SceneView& sv, const RayF& r, void* region, 
   uint32_t* cb = reinterpret_cast<uint32_t*>(ct + kCoopMaxCross + 1);
   uint32_t* co = cb + kCoopMaxCross + 1;
   uint2* jobs = reinterpret_cast<uint2*>(co + kCoopMaxCross + 1);  // kCoopJobs primitive tests
-  uint32_t nCross = 0, nEntered = 0;  // wave-uniform
+  uint32_t nCross = 0, nEntered = 0, pend = 0;  // wave-uniform
   // A: the walk down.  Each iteration enters the operators of the lanes'
   // tasks (box tests, spawns), hands out the next iteration's tasks and
-  // starts loading their records, then tests the hit primitive children
-  // gathered from all lanes, one a lane, and appends their crossings.
+  // starts loading their records, and appends the hit primitive children
+  // of all lanes to the pending tests; whole batches of 64 of those, and at
+  // the end of the walk the rest, are tested one a lane and append their
+  // crossings (the crossings' order is irrelevant: phase B sorts them).
   uint32_t mine = lane == 0 ? 0u : kNone;  // ENTER of the root, flip parity 0
   uint32_t head = 0, tail = 0;
   float4 q0 = make_float4(0, 0, 0, 0), q1 = q0, q2 = q0, q3 = q0;
@@ -1532,12 +1534,12 @@ __device__ bool IntervalTrace(const SceneView& sv, const RayF& r, void* region, 
     tail += static_cast<uint32_t>(__popcll(bq));
     const uint64_t bl = __ballot(jobL != kNone), bR = __ballot(jobR != kNone);
     const uint32_t nl = static_cast<uint32_t>(__popcll(bl));
-    const uint32_t nJobs = nl + static_cast<uint32_t>(__popcll(bR));
-    if (jobL != kNone) jobs[LanesBelow(bl)] = make_uint2(jobL, dL);
-    if (jobR != kNone) jobs[nl + LanesBelow(bR)] = make_uint2(jobR, dR);
+    if (jobL != kNone) jobs[pend + LanesBelow(bl)] = make_uint2(jobL, dL);
+    if (jobR != kNone) jobs[pend + nl + LanesBelow(bR)] = make_uint2(jobR, dR);
+    pend += nl + static_cast<uint32_t>(__popcll(bR));  // < 64 + 128 = kCoopJobs
     WaveSync();
     // next iteration's tasks: the first spawn stays, idle lanes take queued
-    // ones in lane order; their records load during the primitive tests
+    // ones in lane order; their records load during any primitive tests
     mine = sp0;
     const uint64_t idle = __ballot(mine == kNone);
     const uint32_t rank = LanesBelow(idle);
@@ -1552,11 +1554,13 @@ __device__ bool IntervalTrace(const SceneView& sv, const RayF& r, void* region, 
       q2 = sv.ops[4 * Y + 2];
       q3 = sv.ops[4 * Y + 3];
     }
+    const bool last = __ballot(mine != kNone) == 0;  // every hit operator entered
     bool bad = false, full = false;
-    for (uint32_t base = 0; base < nJobs; base += 64) {
+    uint32_t base = 0;
+    for (; last ? base < pend : pend - base >= 64; base += 64) {
       uint32_t c = 0, owner = 0;
       Hit a = MissRecord(), b = MissRecord();
-      if (base + lane < nJobs) {
+      if (base + lane < pend) {
         const uint2 jb = jobs[base + lane];
         owner = jb.x;
         c = PrimCrossings(sv, r, jb.y, a, b);
@@ -1585,8 +1589,17 @@ __device__ bool IntervalTrace(const SceneView& sv, const RayF& r, void* region, 
       nCross += total;
     }
     if (full || __ballot(bad) != 0) return false;
+    if (last) break;
+    if (base != 0) {
+      // the untested rest (fewer than 64) moves to the front
+      const uint32_t rest = pend - base;
+      uint2 v = make_uint2(0u, 0u);
+      if (static_cast<uint32_t>(lane) < rest) v = jobs[base + lane];
+      WaveSync();
+      if (static_cast<uint32_t>(lane) < rest) jobs[lane] = v;
+      pend = rest;
+    }
     WaveSync();
-    if (__ballot(mine != kNone) == 0) break;  // every hit operator entered
   }
   if (iter >= kIvMaxIters) return false;  // stamps would overflow (not reached by trees of 32767 operators)
   // B: sort the crossings, one a lane, and mark each primitive's intervals.
