@@ -49,7 +49,8 @@ class Outputs(ctypes.Structure):
 class SceneInfo(ctypes.Structure):
     _fields_ = [("nodes", ctypes.c_int32), ("primitives", ctypes.c_int32), ("op_depth", ctypes.c_int32),
                 ("spheres", ctypes.c_int32), ("cylinders", ctypes.c_int32), ("cubes", ctypes.c_int32),
-                ("device_bytes", ctypes.c_int64), ("prims_in_boxes", ctypes.c_int32)]
+                ("device_bytes", ctypes.c_int64), ("prims_in_boxes", ctypes.c_int32),
+                ("union_groups", ctypes.c_int32)]
 
 
 class RenderOpts(ctypes.Structure):

@@ -40,9 +40,9 @@ size_t Align256(size_t n) { return (n + 255) & ~static_cast<size_t>(255); }
 // One device allocation holding the scene arrays back to back.
 csg_status Upload(csg_scene* s) {
   const csgray::FlatScene& f = s->flat;
-  const std::vector<csgray::F4>* src[5] = {&f.ops, &f.primA, &f.primB, &f.primC, &f.ivOps};
-  size_t off[5], total = 0;
-  for (int k = 0; k < 5; k++) {
+  const std::vector<csgray::F4>* src[6] = {&f.ops, &f.primA, &f.primB, &f.primC, &f.ivOps, &f.walkOps};
+  size_t off[6], total = 0;
+  for (int k = 0; k < 6; k++) {
     off[k] = total;
     total += Align256(sizeof(csgray::F4) * src[k]->size());
   }
@@ -55,7 +55,7 @@ csg_status Upload(csg_scene* s) {
     return e == hipErrorOutOfMemory ? Fail(CSG_E_OOM, "hipMalloc of the scene failed") : FailHip(e, "hipMalloc");
   }
   char* base = static_cast<char*>(s->dmem);
-  for (int k = 0; k < 5; k++) {
+  for (int k = 0; k < 6; k++) {
     if (src[k]->empty()) continue;
     e = hipMemcpy(base + off[k], src[k]->data(), sizeof(csgray::F4) * src[k]->size(), hipMemcpyHostToDevice);
     if (e != hipSuccess) return FailHip(e, "hipMemcpy of the scene");
@@ -65,6 +65,7 @@ csg_status Upload(csg_scene* s) {
   s->dev.primB = reinterpret_cast<const float4*>(base + off[2]);
   s->dev.primC = reinterpret_cast<const float4*>(base + off[3]);
   s->dev.ivOps = f.ivOps.empty() ? nullptr : reinterpret_cast<const float4*>(base + off[4]);
+  s->dev.walkOps = f.walkOps.empty() ? nullptr : reinterpret_cast<const float4*>(base + off[5]);
   s->dev.rootDesc = f.rootDesc;
   s->dev.nOps = f.nOps;
   s->dev.nPrims = f.nPrims;
@@ -311,6 +312,7 @@ csg_status csg_scene_info_get(const csg_scene* s, csg_scene_info* info) {
   }
   info->device_bytes = static_cast<int64_t>(s->bytes);
   info->prims_in_boxes = s->flat.conservative ? 1 : 0;
+  info->union_groups = s->flat.unionGroups;
   return CSG_OK;
 }
 

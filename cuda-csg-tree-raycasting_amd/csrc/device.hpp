@@ -21,6 +21,7 @@ struct DevScene {
   int32_t opLeaves;  // operators without an operator child: a ray's live cooperative tasks never exceed it
   uint32_t conservative;  // 1: every primitive lies inside its box (interval classification applies)
   const float4* ivOps;    // operator records with fast mode's culling boxes (tight where the reference allows)
+  const float4* walkOps;  // interval classification's records: ivOps with Union groups flattened (scene.hpp)
 };
 
 // One render job: a scene seen by one camera at one resolution, with its

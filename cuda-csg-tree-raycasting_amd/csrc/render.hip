@@ -1386,15 +1386,23 @@ __device__ Hit CoopTrace(const SceneView& sv, const RayF& r, OpState* st, uint16
 //      spawns for hit operator children) collects the crossings > 0 of every
 //      hit primitive child: the reference's own intersection functions at
 //      tmin 0 and again at the first crossing, so times and hit bits are the
-//      state machine's;
+//      state machine's.  A Union group (a Union subtree flattened by the host
+//      into its members, the nodes under its Unions that are not Unions) is
+//      one ENTER: the whole wave tests its members' boxes, 64 a round.  A
+//      Union's solid is the union of its members' however the Unions pair
+//      them, and no member's flip parity changes inside the group, so the
+//      group's mask is the or of its members' masks: one walk iteration for
+//      the group where the tree takes one a level;
 //   B. the wave sorts the k <= 31 crossings by time (one per lane, each placed by a count);
 //      interval i of the ray is the stretch before crossing i (i = k: after
-//      the last), and each primitive's membership over the intervals becomes
-//      a 32-bit mask (bits after its Enter, up to its Exit);
+//      the last), each crossed primitive's membership over the intervals
+//      becomes a 32-bit mask of its own (bits after its Enter, up to its
+//      Exit), and the masks are or'ed into their owners' child slots;
 //   C. the operators combine their children's masks bottom-up (Union: or,
 //      Intersection: and, Difference: and-not), all operators entered in one
 //      walk iteration at once, from the last iteration back (a child is
-//      entered after its parent, so its mask is ready first).
+//      entered after its parent, so its mask is ready first), or'ing the
+//      result into the parent's slot.
 // The answer is the first crossing at which the root's mask changes; its hit
 // bits are the primitive's, with Enter and Exit swapped and Flip toggled once
 // per Difference whose right subtree holds it (the state machine's flip of a
@@ -1404,8 +1412,7 @@ __device__ Hit CoopTrace(const SceneView& sv, const RayF& r, OpState* st, uint16
 // crossings in the same order, so it returns that crossing.  The work is one
 // pass over the hit part of the tree, where the machine's loops re-enter
 // subtrees once per crossing they pass (about 1,000 steps for each of the
-// heaviest 8 % of the 1920x1080 cheese frame's rays, against about 300 box
-// pairs and 40 primitive tests here).
+// heaviest 8 % of the 1920x1080 cheese frame's rays).
 //
 // The ray goes to the state machine instead (CoopTrace from scratch) when
 // the two could differ or the masks cannot hold it: two crossings of equal
@@ -1418,13 +1425,17 @@ __device__ Hit CoopTrace(const SceneView& sv, const RayF& r, OpState* st, uint16
 // two child masks (8 B) and a word: parent (bits 0-14, 0x7FFF for the root),
 // operator (15-16), right child of its parent (17), the walk iteration that
 // entered it (18-31); then the entered operators in order (16 bits each), the
-// crossings (time, hit bits, owner: operator << 2 | side << 1 | flip parity),
-// one iteration's primitive tests (owner, descriptor), and the task ring
-// (ENTER tasks: operator | flip parity << 15) at the region's end.
+// crossings (time, hit bits, tag: owner = operator << 2 | side << 1 | flip
+// parity, the crossed primitive's number << 24, 1 << 29 for an Exit alone),
+// the pending primitive tests (owner, descriptor; in B the crossed
+// primitives' masks), and the task ring (ENTER tasks: operator | flip parity
+// << 15) at the region's end.
 constexpr uint32_t kOpUnion = 0, kOpDifference = 1;  // operator numbering (CSGTree.cuh:38-46)
 constexpr uint32_t kIvFlip = kFlip | kExit | kEnter;
 constexpr uint32_t kIvRoot = 0x7FFFu, kIvStampShift = 18;
 constexpr uint32_t kIvMaxIters = 1u << 14;  // walk iterations the 14-bit stamps number
+constexpr uint32_t kIvGroup = 0x200u;       // type word bit of a Union group's record (scene.hpp: kGroupFlag)
+constexpr uint32_t kIvPrimShift = 24, kIvLone = 1u << 29;
 
 // A hit primitive's crossings beyond 0, as the reference's intersection
 // function reports them queried at tmin 0 and again at the first crossing
@@ -1467,7 +1478,8 @@ __device__ __forceinline__ uint32_t PrimCrossings(const SceneView& sv, const Ray
   return (Flags(b.bits) & kMissBit) ? 1u : 2u;
 }
 
-// Returns false when the ray must go to the state machine instead.
+// Returns false when the ray must go to the state machine instead.  `sv.ops`
+// are the walk's records (DevScene::walkOps).
 __device__ bool IntervalTrace(const SceneView& sv, const RayF& r, void* region, int32_t ops, uint16_t* q,
                               uint32_t qmask, int lane, Hit& res) {
   uint32_t* msk = reinterpret_cast<uint32_t*>(region);
@@ -1476,14 +1488,63 @@ __device__ bool IntervalTrace(const SceneView& sv, const RayF& r, void* region, 
   float* ct = reinterpret_cast<float*>(static_cast<char*>(region) + CoopCrossOffset(ops));
   uint32_t* cb = reinterpret_cast<uint32_t*>(ct + kCoopMaxCross + 1);
   uint32_t* co = cb + kCoopMaxCross + 1;
-  uint2* jobs = reinterpret_cast<uint2*>(co + kCoopMaxCross + 1);  // kCoopJobs primitive tests
-  uint32_t nCross = 0, nEntered = 0, pend = 0;  // wave-uniform
+  uint2* jobs = reinterpret_cast<uint2*>(co + kCoopMaxCross + 1);  // kCoopJobs pending primitive tests
+  uint32_t nCross = 0, nEntered = 0, pend = 0, nPrims = 0;  // wave-uniform
+  // Tests the pending primitive tests from the first, 64 a batch, while more
+  // than `keep` are pending, appends their crossings (in any order: B sorts
+  // them), numbering each crossed primitive for its mask, and moves the
+  // untested rest (fewer than 64) to the front.  False: the ray goes to the
+  // state machine.
+  auto drain = [&](uint32_t keep) -> bool {
+    uint32_t base = 0;
+    bool bad = false;
+    for (; base < pend && pend - base > keep; base += 64) {
+      uint32_t c = 0, owner = 0;
+      Hit a = MissRecord(), b = MissRecord();
+      if (base + lane < pend) {
+        const uint2 jb = jobs[base + lane];
+        owner = jb.x;
+        c = PrimCrossings(sv, r, jb.y, a, b);
+        // (Enter, Exit) or (Exit) alone
+        if (c == 2) bad |= !(a.bits & kEnter) || !(b.bits & kExit);
+        else if (c == 1) bad |= !(a.bits & kExit);
+      }
+      const uint64_t b1 = __ballot(c >= 1), b2 = __ballot(c == 2);
+      const uint32_t total = static_cast<uint32_t>(__popcll(b1) + __popcll(b2));
+      if (nCross + total > kCoopMaxCross) return false;
+      const uint32_t tag = owner | ((nPrims + LanesBelow(b1)) << kIvPrimShift) | (c == 1 ? kIvLone : 0u);
+      const uint32_t at = nCross + LanesBelow(b1) + LanesBelow(b2);
+      if (c >= 1) {
+        ct[at] = a.t;
+        cb[at] = a.bits;
+        co[at] = tag;
+      }
+      if (c == 2) {
+        ct[at + 1] = b.t;
+        cb[at + 1] = b.bits;
+        co[at + 1] = tag;
+      }
+      nCross += total;
+      nPrims += static_cast<uint32_t>(__popcll(b1));
+    }
+    if (__ballot(bad) != 0) return false;
+    if (base != 0) {
+      const uint32_t rest = base < pend ? pend - base : 0u;  // <= keep < 64
+      uint2 v = make_uint2(0u, 0u);
+      if (static_cast<uint32_t>(lane) < rest) v = jobs[base + lane];
+      WaveSync();
+      if (static_cast<uint32_t>(lane) < rest) jobs[lane] = v;
+      pend = rest;
+    }
+    WaveSync();
+    return true;
+  };
   // A: the walk down.  Each iteration enters the operators of the lanes'
-  // tasks (box tests, spawns), hands out the next iteration's tasks and
-  // starts loading their records, and appends the hit primitive children
-  // of all lanes to the pending tests; whole batches of 64 of those, and at
-  // the end of the walk the rest, are tested one a lane and append their
-  // crossings (the crossings' order is irrelevant: phase B sorts them).
+  // tasks (box tests, spawns) and adds their hit primitive children to the
+  // pending tests; the wave then tests the members of the Union groups
+  // entered, hands out the next iteration's tasks, starts loading their
+  // records, and tests whole batches of pending primitives (at the end of the
+  // walk, the rest).
   uint32_t mine = lane == 0 ? 0u : kNone;  // ENTER of the root, flip parity 0
   uint32_t head = 0, tail = 0;
   float4 q0 = make_float4(0, 0, 0, 0), q1 = q0, q2 = q0, q3 = q0;
@@ -1495,40 +1556,43 @@ __device__ bool IntervalTrace(const SceneView& sv, const RayF& r, void* region, 
   }
   uint32_t iter = 0;
   for (; iter < kIvMaxIters; iter++) {
-    uint32_t sp0 = kNone, sp1 = kNone, jobL = kNone, jobR = kNone, dL = 0, dR = 0;
+    uint32_t sp0 = kNone, sp1 = kNone, jobL = kNone, jobR = kNone, dL = 0, dR = 0, grp = kNone;
     if (mine != kNone) {
       const uint32_t X = mine & 0x7FFFu, par = mine >> 15;
       const uint32_t lDesc = AsU(q3.x), rDesc = AsU(q3.y), typeWord = AsU(q3.z), parent = AsU(q3.w);
       const uint32_t op = typeWord & 0xFFu;
-      const bool gL = BoxHit<true>(r, q0.x, q0.y, q0.z, q1.x, q1.y, q1.z, 0.0f);
-      const bool gR = BoxHit<true>(r, q0.w, q1.w, q2.x, q2.y, q2.z, q2.w, 0.0f);
-      const bool oL = gL & (DescType(lDesc) < kSphereT), oR = gR & (DescType(rDesc) < kSphereT);
-      const uint32_t parR = par ^ (op == kOpDifference ? 1u : 0u);
-      if (gL & !oL) {
-        jobL = (X << 2) | par;
-        dL = lDesc;
-      } else if (!oL) {
-        msk[2 * X] = 0u;
-      }
-      if (gR & !oR) {
-        jobR = (X << 2) | 2u | parR;
-        dR = rDesc;
-      } else if (!oR) {
-        msk[2 * X + 1] = 0u;
-      }
+      // both child slots start empty; the children's masks are or'ed in
+      *reinterpret_cast<uint2*>(msk + 2 * X) = make_uint2(0u, 0u);
       meta[X] = (parent == kNone ? kIvRoot : parent) | (op << 15) | (((typeWord >> 8) & 1u) << 17) |
                 (iter << kIvStampShift);
-      if (oL) sp0 = DescIndex(lDesc) | (par << 15);
-      if (oR) {
-        if (oL) sp1 = DescIndex(rDesc) | (parR << 15);
-        else sp0 = DescIndex(rDesc) | (parR << 15);
+      if (typeWord & kIvGroup) {
+        grp = mine;  // a Union group: the wave tests its members below
+      } else {
+        const bool gL = BoxHit<true>(r, q0.x, q0.y, q0.z, q1.x, q1.y, q1.z, 0.0f);
+        const bool gR = BoxHit<true>(r, q0.w, q1.w, q2.x, q2.y, q2.z, q2.w, 0.0f);
+        const bool oL = gL & (DescType(lDesc) < kSphereT), oR = gR & (DescType(rDesc) < kSphereT);
+        const uint32_t parR = par ^ (op == kOpDifference ? 1u : 0u);
+        if (gL & !oL) {
+          jobL = (X << 2) | par;
+          dL = lDesc;
+        }
+        if (gR & !oR) {
+          jobR = (X << 2) | 2u | parR;
+          dR = rDesc;
+        }
+        if (oL) sp0 = DescIndex(lDesc) | (par << 15);
+        if (oR) {
+          if (oL) sp1 = DescIndex(rDesc) | (parR << 15);
+          else sp0 = DescIndex(rDesc) | (parR << 15);
+        }
       }
     }
     // the entered operators, in order (stamped with this iteration in meta)
     const uint64_t be = __ballot(mine != kNone);
     if (mine != kNone) entered[nEntered + LanesBelow(be)] = static_cast<uint16_t>(mine & 0x7FFFu);
     nEntered += static_cast<uint32_t>(__popcll(be));
-    // queue the second spawns, gather the primitive tests
+    // queue the second spawns, gather the primitive tests (fewer than 64
+    // were pending: at most 191 now)
     const uint64_t bq = __ballot(sp1 != kNone);
     if (sp1 != kNone) q[(tail + LanesBelow(bq)) & qmask] = static_cast<uint16_t>(sp1);
     tail += static_cast<uint32_t>(__popcll(bq));
@@ -1536,8 +1600,42 @@ __device__ bool IntervalTrace(const SceneView& sv, const RayF& r, void* region, 
     const uint32_t nl = static_cast<uint32_t>(__popcll(bl));
     if (jobL != kNone) jobs[pend + LanesBelow(bl)] = make_uint2(jobL, dL);
     if (jobR != kNone) jobs[pend + nl + LanesBelow(bR)] = make_uint2(jobR, dR);
-    pend += nl + static_cast<uint32_t>(__popcll(bR));  // < 64 + 128 = kCoopJobs
+    pend += nl + static_cast<uint32_t>(__popcll(bR));
     WaveSync();
+    // the Union groups entered: members' boxes 64 a round; hit primitives
+    // join the pending tests as the group's left child, hit operators are
+    // queued (the pending tests drop below 64 before each round)
+    for (uint64_t bg = __ballot(grp != kNone); bg != 0; bg &= bg - 1) {
+      const int gl = __ffsll(static_cast<unsigned long long>(bg)) - 1;
+      const uint32_t g = __builtin_amdgcn_readlane(grp, gl);
+      const uint32_t first = __builtin_amdgcn_readlane(AsU(q3.x), gl);
+      const uint32_t count = __builtin_amdgcn_readlane(AsU(q3.y), gl);
+      const uint32_t owner = ((g & 0x7FFFu) << 2) | (g >> 15);
+      for (uint32_t m = 0; m < count; m += 64) {
+        if (!drain(63)) return false;
+        uint32_t job = kNone, d = 0, spawn = kNone;
+        if (m + lane < count) {
+          const size_t at = first + 2 * static_cast<size_t>(m + lane);
+          const float4 m0 = sv.ops[at], m1 = sv.ops[at + 1];
+          const uint32_t md = AsU(m1.z);
+          if (BoxHit<true>(r, m0.x, m0.y, m0.z, m0.w, m1.x, m1.y, 0.0f)) {
+            if (DescType(md) < kSphereT) {
+              spawn = DescIndex(md) | (g & 0x8000u);
+            } else {
+              job = owner;
+              d = md;
+            }
+          }
+        }
+        const uint64_t bs = __ballot(spawn != kNone);
+        if (spawn != kNone) q[(tail + LanesBelow(bs)) & qmask] = static_cast<uint16_t>(spawn);
+        tail += static_cast<uint32_t>(__popcll(bs));
+        const uint64_t bj = __ballot(job != kNone);
+        if (job != kNone) jobs[pend + LanesBelow(bj)] = make_uint2(job, d);
+        pend += static_cast<uint32_t>(__popcll(bj));
+        WaveSync();
+      }
+    }
     // next iteration's tasks: the first spawn stays, idle lanes take queued
     // ones in lane order; their records load during any primitive tests
     mine = sp0;
@@ -1554,52 +1652,10 @@ __device__ bool IntervalTrace(const SceneView& sv, const RayF& r, void* region, 
       q2 = sv.ops[4 * Y + 2];
       q3 = sv.ops[4 * Y + 3];
     }
-    const bool last = __ballot(mine != kNone) == 0;  // every hit operator entered
-    bool bad = false, full = false;
-    uint32_t base = 0;
-    for (; last ? base < pend : pend - base >= 64; base += 64) {
-      uint32_t c = 0, owner = 0;
-      Hit a = MissRecord(), b = MissRecord();
-      if (base + lane < pend) {
-        const uint2 jb = jobs[base + lane];
-        owner = jb.x;
-        c = PrimCrossings(sv, r, jb.y, a, b);
-        // (Enter, Exit) or (Exit) alone
-        if (c == 2) bad |= !(a.bits & kEnter) || !(b.bits & kExit);
-        else if (c == 1) bad |= !(a.bits & kExit);
-        msk[2 * (owner >> 2) + ((owner >> 1) & 1u)] = (c == 1) ? ~0u : 0u;  // inside from 0: an Exit alone
-      }
-      const uint64_t b1 = __ballot(c >= 1), b2 = __ballot(c == 2);
-      const uint32_t total = static_cast<uint32_t>(__popcll(b1) + __popcll(b2));
-      if (nCross + total > kCoopMaxCross) {
-        full = true;
-        break;
-      }
-      const uint32_t at = nCross + LanesBelow(b1) + LanesBelow(b2);
-      if (c >= 1) {
-        ct[at] = a.t;
-        cb[at] = a.bits;
-        co[at] = owner;
-      }
-      if (c == 2) {
-        ct[at + 1] = b.t;
-        cb[at + 1] = b.bits;
-        co[at + 1] = owner;
-      }
-      nCross += total;
-    }
-    if (full || __ballot(bad) != 0) return false;
+    // no task left (an idle lane would have taken any queued one): the walk is done
+    const bool last = __ballot(mine != kNone) == 0;
+    if (!drain(last ? 0u : 63u)) return false;
     if (last) break;
-    if (base != 0) {
-      // the untested rest (fewer than 64) moves to the front
-      const uint32_t rest = pend - base;
-      uint2 v = make_uint2(0u, 0u);
-      if (static_cast<uint32_t>(lane) < rest) v = jobs[base + lane];
-      WaveSync();
-      if (static_cast<uint32_t>(lane) < rest) jobs[lane] = v;
-      pend = rest;
-    }
-    WaveSync();
   }
   if (iter >= kIvMaxIters) return false;  // stamps would overflow (not reached by trees of 32767 operators)
   // B: sort the crossings, one a lane, and mark each primitive's intervals.
@@ -1609,11 +1665,11 @@ __device__ bool IntervalTrace(const SceneView& sv, const RayF& r, void* region, 
   // to the state machine.
   const uint32_t k = nCross;
   float key = __builtin_inff();
-  uint32_t bits = 0u, owner = 0u;
+  uint32_t bits = 0u, tag = 0u;
   if (static_cast<uint32_t>(lane) < k) {
     key = ct[lane];
     bits = cb[lane];
-    owner = co[lane];
+    tag = co[lane];
   }
   uint32_t place = 0;
   bool tie = false;
@@ -1626,19 +1682,28 @@ __device__ bool IntervalTrace(const SceneView& sv, const RayF& r, void* region, 
   if (static_cast<uint32_t>(lane) < k) {
     ct[place] = key;
     cb[place] = bits;
-    co[place] = owner;
+    co[place] = tag;
   }
   WaveSync();
   if (static_cast<uint32_t>(lane) < k) {
     key = ct[lane];
     bits = cb[lane];
-    owner = co[lane];
+    tag = co[lane];
   }
+  // each crossed primitive's mask (the pending tests' space is free now):
+  // set after its Enter, clear after its Exit, all set up to an Exit alone;
+  // then or'ed into its owner's slot by the lane of its first crossing
   const bool held = static_cast<uint32_t>(lane) < k;
-  const uint32_t slot = 2 * (owner >> 2) + ((owner >> 1) & 1u);
-  if (held && (bits & kEnter)) atomicOr(&msk[slot], ~0u << (lane + 1));
+  uint32_t* pm = reinterpret_cast<uint32_t*>(jobs);
+  const uint32_t pi = (tag >> kIvPrimShift) & 31u;
+  const bool firstX = held && ((bits & kEnter) || (tag & kIvLone));
+  if (firstX) pm[pi] = (bits & kEnter) ? 0u : ~0u;
   WaveSync();
-  if (held && !(bits & kEnter)) atomicAnd(&msk[slot], (2u << lane) - 1u);
+  if (held && (bits & kEnter)) atomicOr(&pm[pi], ~0u << (lane + 1));
+  WaveSync();
+  if (held && !(bits & kEnter)) atomicAnd(&pm[pi], (2u << lane) - 1u);
+  WaveSync();
+  if (firstX) atomicOr(&msk[2 * ((tag >> 2) & 0x7FFFu) + ((tag >> 1) & 1u)], pm[pi]);
   WaveSync();
   // C: combine masks bottom-up, the operators entered in one iteration at a
   // time from the last iteration back (a child is always entered after its
@@ -1660,7 +1725,7 @@ __device__ bool IntervalTrace(const SceneView& sv, const RayF& r, void* region, 
         rootMask = m;
         done = true;
       } else {
-        msk[2 * parent + ((md >> 17) & 1u)] = m;
+        atomicOr(&msk[2 * parent + ((md >> 17) & 1u)], m);  // a group's members share its left slot
       }
     }
     pos -= static_cast<uint32_t>(__popcll(__ballot(grp)));
@@ -1676,7 +1741,7 @@ __device__ bool IntervalTrace(const SceneView& sv, const RayF& r, void* region, 
     return true;
   }
   const int j = __ffs(change) - 1;
-  const uint32_t hb = bits ^ ((owner & 1u) ? kIvFlip : 0u);
+  const uint32_t hb = bits ^ ((tag & 1u) ? kIvFlip : 0u);
   res.t = __uint_as_float(__builtin_amdgcn_readlane(AsU(key), j));
   res.bits = __builtin_amdgcn_readlane(hb, j);
   return true;
@@ -1772,8 +1837,8 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) Co
     Hit h = MissRecord();
     bool listed = false;
     if (kFast && !kCount && p.coopLists && jb.sc.conservative) {
-      // the walk culls with boxes tight around each subtree's solid
-      const SceneView tv{jb.sc.ivOps ? jb.sc.ivOps : jb.sc.ops, jb.sc.primA, jb.sc.primB, jb.sc.ops, 0u};
+      // the walk's records: boxes tight around each subtree's solid, Union groups flattened
+      const SceneView tv{jb.sc.walkOps ? jb.sc.walkOps : jb.sc.ops, jb.sc.primA, jb.sc.primB, jb.sc.ops, 0u};
       listed = IntervalTrace(tv, r, ldsBase, p.coopOps, q, qmask, lane, h);
     }
     if (listed) listedRays++;

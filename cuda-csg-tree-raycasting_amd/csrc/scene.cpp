@@ -375,6 +375,63 @@ FlatScene Flatten(const CSGTree& tree) {
       q[1] = F4{L.hi[0], L.hi[1], L.hi[2], R.lo[1]};
       q[2] = F4{R.lo[2], R.hi[0], R.hi[1], R.hi[2]};
     }
+    // Union groups for the interval walk (layout in scene.hpp).  A Union's
+    // solid is the union of its members' however the Unions pair them, and
+    // a member's flip parity (Differences whose right subtree holds it) does
+    // not change inside the group, so the walk may take the group as one
+    // operator over its members.
+    f.walkOps = f.ivOps;
+    std::vector<int32_t> members, todo;
+    std::vector<int32_t> roots;
+    for (int32_t i = 0; i < f.nNodes; i++) {
+      const CSGNode& n = tree.nodes[i];
+      if (n.primitiveIdx == -1 && n.type == kUnion && (n.parent < 0 || tree.nodes[n.parent].type != kUnion))
+        roots.push_back(i);
+    }
+    while (!roots.empty()) {
+      const int32_t g = roots.back();
+      roots.pop_back();
+      members.clear();
+      todo.assign(1, g);
+      while (!todo.empty()) {
+        const int32_t k = todo.back();
+        todo.pop_back();
+        const CSGNode& n = tree.nodes[k];
+        for (const int32_t c : {n.right, n.left}) {
+          const CSGNode& m = tree.nodes[c];
+          if (m.primitiveIdx == -1 && m.type == kUnion) todo.push_back(c);
+          else members.push_back(c);
+        }
+      }
+      if (members.size() > static_cast<size_t>(kMaxGroupMembers)) {
+        // too many for one wave's rounds: keep this Union binary, try its Union children
+        for (const int32_t c : {tree.nodes[g].left, tree.nodes[g].right}) {
+          if (tree.nodes[c].primitiveIdx == -1 && tree.nodes[c].type == kUnion) roots.push_back(c);
+        }
+        continue;
+      }
+      if (members.size() < 3) continue;
+      const uint32_t gi = static_cast<uint32_t>(opIndex[g]);
+      const size_t first = f.walkOps.size();
+      for (const int32_t m : members) {
+        const Box& b = tight[m];
+        f.walkOps.push_back(F4{b.lo[0], b.lo[1], b.lo[2], b.hi[0]});
+        f.walkOps.push_back(F4{b.hi[1], b.hi[2], bits(desc(m)), 0.0f});
+        if (tree.nodes[m].primitiveIdx == -1) {
+          F4& w = f.walkOps[4 * static_cast<size_t>(opIndex[m]) + 3];
+          uint32_t tw;
+          std::memcpy(&tw, &w.z, 4);
+          w.z = bits(tw & ~0x100u);
+          w.w = bits(gi);
+        }
+      }
+      F4& w = f.walkOps[4 * static_cast<size_t>(gi) + 3];
+      uint32_t tw;
+      std::memcpy(&tw, &w.z, 4);
+      w = F4{bits(static_cast<uint32_t>(first)), bits(static_cast<uint32_t>(members.size())), bits(tw | kGroupFlag),
+             w.w};
+      f.unionGroups++;
+    }
   }
   return f;
 }

@@ -77,6 +77,18 @@ struct CSGTree {
 //               (right = 1 when the operator is its parent's right child)
 //   child descriptor = node type | (index << 8), index = primitive index for a
 //   primitive child, operator index for an operator child
+//
+// The interval walk's records (walkOps) are the operator records with fast
+// mode's culling boxes, except that each Union group -- a Union whose parent
+// is not a Union, with the Unions below it, of 3 to kMaxGroupMembers members
+// (the nodes under those Unions that are not Unions) -- is flattened: the
+// group's record holds bits(first member record, as a float4 index into
+// walkOps), bits(member count), type | right<<8 | kGroupFlag, parent; each
+// member is two float4s after the operator records,
+//   (lo.x, lo.y, lo.z, hi.x), (hi.y, hi.z, bits(desc), 0),
+// with its culling box; a member that is an operator names the group as its
+// parent (as a left child).  A larger Union subtree stays binary at the top
+// and its Union children are tried as groups.
 //   primA[p] = (centre.xyz, radius or cube edge)
 //   primB[p] = (axis.xyz, height)
 //   primC[p] = (r, g, b, 0)
@@ -90,11 +102,18 @@ struct FlatScene {
   // each child's solid where the reference's culling allows; Flatten); empty
   // for a scene without operators.
   std::vector<F4> ivOps;
+  // The interval walk's records: ivOps with each Union group flattened, then
+  // the groups' member records (layout below); empty without operators.
+  std::vector<F4> walkOps;
+  int32_t unionGroups = 0;  // Union groups flattened in walkOps
   uint32_t rootDesc = 0;
   int32_t nNodes = 0, nOps = 0, nPrims = 0, opDepth = 0;
   int32_t opLeaves = 0;  // operators without an operator child
   bool conservative = true;  // every primitive lies inside its box (false: a tilted cylinder pokes out)
 };
+
+constexpr uint32_t kGroupFlag = 0x200u;    // type word bit of a flattened Union group (walkOps)
+constexpr int32_t kMaxGroupMembers = 1024;  // a group's members are tested 64 a round by one wave
 
 constexpr int32_t kMaxNodes = 65535;       // 16-bit node links
 constexpr int32_t kMaxPrimitives = 32767;  // the reference stores hit ids as int16 (Ray.cuh:41)

@@ -93,6 +93,8 @@ typedef struct csg_scene_info {
   int32_t prims_in_boxes;  /* 1: every primitive lies inside its box (a cylinder tilted off the
                               coordinate axes may not: the reference's box is centre +- max(h/2, r)),
                               so handed-over rays may be classified by intervals */
+  int32_t union_groups;    /* Union subtrees (3 to 1024 members below their Unions) that interval
+                              classification walks as one operator, testing all members' boxes at once */
 } csg_scene_info;
 
 /* Arithmetic of the traversal kernel.

@@ -145,6 +145,50 @@ def test_interval_classification_on_heavy_scenes(gpu, make):
     assert handed > 0 and classified > 0.9 * handed
 
 
+def _union_of_operators(n=96, seed=11):
+    """A Union group whose members are operators: a balanced Union of n
+    holed balls (Difference of two spheres), every fourth member an
+    Intersection of a sphere and a cube, every eighth a Difference whose left
+    operand is a Union of three spheres (a group of its own under it)."""
+    rng = np.random.default_rng(seed)
+    members = []
+    for i in range(n):
+        c = rng.uniform(-1.0, 1.0, 3)
+        r = rng.uniform(0.15, 0.35)
+        inner = c + rng.uniform(-0.1, 0.1, 3)
+        if i % 8 == 7:
+            balls = [S.sphere(c + rng.uniform(-0.15, 0.15, 3), "F5F500", r * 0.7) for _ in range(3)]
+            members.append("\n".join(["Difference"] + S.balanced("Union", balls) + [S.sphere(inner, "F5F500", 0.5 * r)]))
+        elif i % 4 == 3:
+            members.append("\n".join(["Intersection", S.sphere(c, "E0E0E0", r), S.cube(c + 0.05, "E0E0E0", 1.5 * r)]))
+        else:
+            members.append("\n".join(["Difference", S.sphere(c, "F5F500", r), S.sphere(inner, "F5F500", 0.6 * r)]))
+    return "\n".join(S.balanced("Union", members)) + "\n"
+
+
+@pytest.mark.parametrize("make,groups,share", [(lambda: _union_of_operators(96, seed=11), 13, 0.9),
+                                               (lambda: S.cheese(2048, seed=3), 2, 0.5)],
+                         ids=["union_of_operators96", "cheese2048_two_groups"])
+def test_interval_classification_across_union_groups(gpu, make, groups, share):
+    """Union groups (flattened for the walk) whose members are operators
+    sharing the group's slot, a group nested under a member, and a
+    2048-sphere Union kept binary at the top over two 1024-member groups:
+    classification, state machine and per-lane kernel agree bit for bit
+    with every ray handed over.  (The 2048-sphere cheese is four times as
+    dense as the benchmark's: about a third of its rays cross more than 31
+    surfaces and go to the state machine.)"""
+    print("\n[case] union groups fast, intervals vs state machine, 480x270", flush=True)
+    sname, cam, light, phong, cam6, l3, ph8 = setups()[1]
+    s = C.Scene(make())
+    assert s.info().union_groups == groups
+    lane, iv, machine, classified, handed = _three_ways(s, cam, light, phong, 480, 270)
+    s.close()
+    print(f"handed={handed} classified={classified} hits={int((lane['prim_id'] >= 0).sum())}")
+    _same(iv, lane)
+    _same(machine, lane)
+    assert handed > 0 and classified > share * handed
+
+
 @pytest.mark.parametrize("eye,target", [((0.05, 0.02, 0.03), (1.0, 0.4, 0.2)), ((0.9, -0.7, 0.2), (-1.0, 0.3, -0.4))],
                          ids=["centre", "near_corner"])
 def test_interval_classification_from_inside_the_solid(gpu, eye, target):

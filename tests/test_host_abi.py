@@ -114,6 +114,28 @@ def test_scene_info_reports_primitives_inside_their_boxes(text, inside):
     assert s.info().prims_in_boxes == inside
 
 
+S1 = "Sphere 0 0 0 FF0000 1\n"
+
+
+@pytest.mark.parametrize("text,groups", [
+    ("Union\n" + S1 * 2, 0),                                    # two members: stays binary
+    ("Union\nUnion\n" + S1 * 3, 1),                            # three members under two Unions
+    ("Intersection\nIntersection\n" + S1 * 3, 0),              # only Unions are flattened
+    ("Union\nDifference\nUnion\n" + S1 + "Union\n" + S1 * 3 + S1, 1),  # the outer Union has 2 members
+    ("Union\nUnion\nDifference\n" + S1 * 2 + S1 + "Union\n" + S1 * 2, 1),  # a Difference member
+])
+def test_scene_info_counts_union_groups(text, groups):
+    assert C.Scene(text, upload=False).info().union_groups == groups
+
+
+def test_union_groups_split_above_1024_members():
+    from csgray import scenes as S
+    assert C.Scene(S.cheese(512, seed=512), upload=False).info().union_groups == 1
+    assert C.Scene(S.cheese(1024, seed=3), upload=False).info().union_groups == 1
+    # 2048 members: the top Union stays binary, its two 1024-member halves are groups
+    assert C.Scene(S.cheese(2048, seed=3), upload=False).info().union_groups == 2
+
+
 def test_render_refuses_host_only_scene_without_touching_the_gpu():
     s = C.Scene(scene_text("test1.txt"), upload=False)
     o = C.Outputs(None, None, None, None)
