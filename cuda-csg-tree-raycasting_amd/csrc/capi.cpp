@@ -169,17 +169,19 @@ CoopPlan PlanCoop(const LaunchConfig& cfg, int32_t maxOps, int32_t maxLeaves, bo
 // batched mix, each in its own launch, 222 -> 215.5 ms, the cheese frames
 // unchanged (a cap of 150 % would cost its 1024x1024 frame 10 %).
 // Where the cooperative kernel classifies rays by intervals (`lists`) a
-// handed-over ray costs it less, and the best grace falls off faster than
-// waves / tiles as frames grow (cheese, medians of five interleaved runs:
-// 1280x720 150-200 %, 1024x1024 200-250 %, 1920x1080 88-90 %, 2560x1440
-// 25-38 %, 3840x2160 7-11 %): 2000 % x (waves / tiles)^1.5, at most 200 %
-// (200, 200, 90, 38 and 11 % for those frames).
+// handed-over ray costs it a fraction of what the per-lane kernel's tail
+// costs, so the tail is handed over at once unless the frame has only a few
+// tiles a wave (cheese, medians of three interleaved runs, budget 64:
+// 3840x2160 and 2560x1440 best with no grace, 1920x1080 with 0-5 %, against
+// 1024x1024 60-150 % and 1280x720 35-90 %; waves / tiles = 0.03, 0.07,
+// 0.13, 0.25 and 0.28): 95000 % x (waves / tiles)^5, at most 100 % (0.5,
+// 0.5, 3, 93 and 100 % for those frames).
 int32_t DefaultCoopGraceMille(int64_t waves, int64_t tiles, bool batched, bool lists) {
   if (tiles <= 0) return 1000;
   if (lists && !batched) {
     const double r = static_cast<double>(waves) / static_cast<double>(tiles);
-    const double g = 20000.0 * r * std::sqrt(r);
-    return static_cast<int32_t>(std::min(2000.0, std::max(5.0, g + 0.5)));
+    const double g = 950000.0 * (r * r) * (r * r) * r;
+    return static_cast<int32_t>(std::min(1000.0, std::max(5.0, g + 0.5)));
   }
   const int64_t k = batched ? 5500 : 9500;
   const int64_t g = (k * waves + tiles / 2) / tiles;

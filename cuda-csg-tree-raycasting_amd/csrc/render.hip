@@ -1602,21 +1602,31 @@ __device__ bool IntervalTrace(const SceneView& sv, const RayF& r, void* region, 
     if (jobR != kNone) jobs[pend + nl + LanesBelow(bR)] = make_uint2(jobR, dR);
     pend += nl + static_cast<uint32_t>(__popcll(bR));
     WaveSync();
-    // the Union groups entered: members' boxes 64 a round; hit primitives
-    // join the pending tests as the group's left child, hit operators are
-    // queued (the pending tests drop below 64 before each round)
+    // the Union groups entered: members' boxes 64 a round (the next round's
+    // records loading meanwhile); hit primitives join the pending tests as
+    // the group's left child, hit operators are queued (the pending tests
+    // drop below 64 before each round)
     for (uint64_t bg = __ballot(grp != kNone); bg != 0; bg &= bg - 1) {
       const int gl = __ffsll(static_cast<unsigned long long>(bg)) - 1;
       const uint32_t g = __builtin_amdgcn_readlane(grp, gl);
       const uint32_t first = __builtin_amdgcn_readlane(AsU(q3.x), gl);
       const uint32_t count = __builtin_amdgcn_readlane(AsU(q3.y), gl);
       const uint32_t owner = ((g & 0x7FFFu) << 2) | (g >> 15);
+      float4 n0 = make_float4(0, 0, 0, 0), n1 = n0;
+      if (static_cast<uint32_t>(lane) < count) {
+        n0 = sv.ops[first + 2 * lane];
+        n1 = sv.ops[first + 2 * lane + 1];
+      }
       for (uint32_t m = 0; m < count; m += 64) {
+        const float4 m0 = n0, m1 = n1;
+        if (m + 64 + lane < count) {
+          const size_t at = first + 2 * static_cast<size_t>(m + 64 + lane);
+          n0 = sv.ops[at];
+          n1 = sv.ops[at + 1];
+        }
         if (!drain(63)) return false;
         uint32_t job = kNone, d = 0, spawn = kNone;
         if (m + lane < count) {
-          const size_t at = first + 2 * static_cast<size_t>(m + lane);
-          const float4 m0 = sv.ops[at], m1 = sv.ops[at + 1];
           const uint32_t md = AsU(m1.z);
           if (BoxHit<true>(r, m0.x, m0.y, m0.z, m0.w, m1.x, m1.y, 0.0f)) {
             if (DescType(md) < kSphereT) {

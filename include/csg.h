@@ -126,17 +126,19 @@ typedef struct csg_render_opts {
   uint32_t seed;        /* jitter seed */
   int32_t tile_walk;    /* 0: row-major tiles; 1: Morton order within a job (square power-of-two grids) */
   int32_t coop_budget;  /* state-machine steps a lane spends on one ray before the ray may be handed to a
-                           whole wave (cooperative evaluation, same results): 0 = default (384; 160 where
+                           whole wave (cooperative evaluation, same results): 0 = default (384; 64 where
                            the wave classifies by intervals, see coop_intervals), < 0 = never */
   int32_t coop_when;    /* 0: hand over only in the frame's tail, once every tile has been taken and a
                            further coop_grace percent of the time that took has passed;
                            1: as soon as the budget is spent */
   int32_t coop_grace;   /* 0 = default: 950 x waves / tiles percent (at most 300) for one scene, 550 x for
                            a batch (the tiles in flight when the queue runs dry finish in about a tile's
-                           time; handing out took tiles / waves tile times), and 2000 x (waves /
-                           tiles)^1.5 (at most 200) for one scene whose rays the wave classifies by
-                           intervals: 120 (90 with intervals) for a 1024-lane launch of 1920x1080, about
-                           3 for the 256-tree batch; < 0 = none; else percent */
+                           time; handing out took tiles / waves tile times), and 95000 x (waves /
+                           tiles)^5 (0.5 to 100) for one scene whose rays the wave classifies by
+                           intervals (such rays cost the wave little: the tail goes over at once
+                           unless the frame has few tiles a wave): 120 (3 with intervals) for a
+                           1024-lane launch of 1920x1080, about 3 for the 256-tree batch; < 0 = none;
+                           else percent */
   int32_t coop_intervals; /* 0 = default: in fast mode, a handed-over ray of a scene whose primitives all
                              lie inside their boxes is classified by intervals (one pass over the hit part
                              of the tree, the crossings sorted, membership masks combined bottom-up; the
@@ -304,8 +306,8 @@ typedef struct csg_launch_info {
   int32_t coop_budget;    /* steps before a ray is handed to the cooperative kernel (0: never) */
   int32_t coop_grid;      /* workgroups of the cooperative kernel (one ray in flight each) */
   int32_t coop_block;     /* its threads per workgroup (64) */
-  int32_t coop_grace;    /* tail grace (percent) a 1920x1080 frame would get (the default is
-                            950 x waves / tiles percent) */
+  int32_t coop_grace;    /* tail grace (percent) a 1920x1080 frame would get (the default of
+                            csg_render_opts.coop_grace) */
   int64_t coop_lds_bytes; /* its dynamic LDS per workgroup */
   int32_t cache_ops;      /* operator records cached in LDS (scene_placement 3), else 0 */
 } csg_launch_info;

@@ -296,19 +296,19 @@ def test_launch_plan(gpu):
             assert c.cache_ops == min(512, (c.lds_limit - 10 * block * 8 - 256) // 64)
     # the largest workgroup whose stacks and scene copy fit, and the default tail grace of a 1920x1080
     # frame (240 x 135 tiles): 950 % x waves / tiles with the state machine in the cooperative kernel
-    # (exact mode), 2000 % x (waves / tiles)^1.5 (at most 200 %) where it classifies by intervals (fast
-    # mode, primitives inside their boxes), with budgets of 384 and 160 steps
+    # (exact mode), 95000 % x (waves / tiles)^5 (at least 0.5 %, at most 100 %) where it classifies by
+    # intervals (fast mode, primitives inside their boxes), with budgets of 384 and 64 steps
     def lists_grace(plan):
         r = plan.grid * (plan.block // 64) / (240 * 135)
-        return (min(2000, int(20000 * r ** 1.5 + 0.5)) + 5) // 10
-    assert auto.scene_in_lds == 1 and auto.block == 1024 and auto.coop_grace == 90 and auto.coop_budget == 160
+        return (min(1000, max(5, int(950000 * r ** 5 + 0.5))) + 5) // 10
+    assert auto.scene_in_lds == 1 and auto.block == 1024 and auto.coop_grace == 3 and auto.coop_budget == 64
     ex = s.plan(mode=C.MODE_EXACT)
     assert ex.coop_grace == 120 and ex.coop_budget == 384
     for plan in (auto, s.plan(block=512), g):
         assert plan.coop_grace == lists_grace(plan)
     for plan in (ex, s.plan(mode=C.MODE_EXACT, block=512)):
         assert plan.coop_grace == round(950 * plan.grid * (plan.block // 64) / (240 * 135))
-    assert s.plan(block=512).coop_grace == 32 and s.plan(mode=C.MODE_EXACT, block=512).coop_grace == 60
+    assert s.plan(block=512).coop_grace == 1 and s.plan(mode=C.MODE_EXACT, block=512).coop_grace == 60
     assert s.plan(coop_grace=30).coop_grace == 30 and s.plan(coop=-1).coop_grace == 0
     # 1024-lane workgroups only for scenes of 256 operators or more: the 64-sphere cheese (64 operators)
     # and the Wikipedia scene (7) take 512 lanes, a 255-sphere cheese 512, a 256-sphere one 1024
