@@ -1602,29 +1602,26 @@ __device__ bool IntervalTrace(const SceneView& sv, const RayF& r, void* region, 
     if (jobR != kNone) jobs[pend + nl + LanesBelow(bR)] = make_uint2(jobR, dR);
     pend += nl + static_cast<uint32_t>(__popcll(bR));
     WaveSync();
-    // the Union groups entered: members' boxes 64 a round (the next round's
-    // records loading meanwhile); hit primitives join the pending tests as
-    // the group's left child, hit operators are queued (the pending tests
-    // drop below 64 before each round)
+    // the Union groups entered: members' boxes 64 a round, two register sets
+    // in turn (one round's records load during the other's tests); hit
+    // primitives join the pending tests as the group's left child, hit
+    // operators are queued (at most 64 tests are pending before each pair of
+    // rounds: room for both)
     for (uint64_t bg = __ballot(grp != kNone); bg != 0; bg &= bg - 1) {
       const int gl = __ffsll(static_cast<unsigned long long>(bg)) - 1;
       const uint32_t g = __builtin_amdgcn_readlane(grp, gl);
       const uint32_t first = __builtin_amdgcn_readlane(AsU(q3.x), gl);
       const uint32_t count = __builtin_amdgcn_readlane(AsU(q3.y), gl);
       const uint32_t owner = ((g & 0x7FFFu) << 2) | (g >> 15);
-      float4 n0 = make_float4(0, 0, 0, 0), n1 = n0;
-      if (static_cast<uint32_t>(lane) < count) {
-        n0 = sv.ops[first + 2 * lane];
-        n1 = sv.ops[first + 2 * lane + 1];
-      }
-      for (uint32_t m = 0; m < count; m += 64) {
-        const float4 m0 = n0, m1 = n1;
-        if (m + 64 + lane < count) {
-          const size_t at = first + 2 * static_cast<size_t>(m + 64 + lane);
-          n0 = sv.ops[at];
-          n1 = sv.ops[at + 1];
-        }
-        if (!drain(63)) return false;
+      // unconditional loads (lanes past the last member read its record and
+      // ignore it), so that the compiler counts them exactly and waits only
+      // for the round it tests
+      auto load = [&](uint32_t m, float4& a, float4& b) {
+        const size_t at = first + 2 * static_cast<size_t>(min(m + static_cast<uint32_t>(lane), count - 1));
+        a = sv.ops[at];
+        b = sv.ops[at + 1];
+      };
+      auto round = [&](uint32_t m, const float4& m0, const float4& m1) {
         uint32_t job = kNone, d = 0, spawn = kNone;
         if (m + lane < count) {
           const uint32_t md = AsU(m1.z);
@@ -1643,8 +1640,21 @@ __device__ bool IntervalTrace(const SceneView& sv, const RayF& r, void* region, 
         const uint64_t bj = __ballot(job != kNone);
         if (job != kNone) jobs[pend + LanesBelow(bj)] = make_uint2(job, d);
         pend += static_cast<uint32_t>(__popcll(bj));
-        WaveSync();
+      };
+      float4 a0 = make_float4(0, 0, 0, 0), a1 = a0, b0 = a0, b1 = a0;
+      load(0, a0, a1);
+      for (uint32_t m = 0; m < count; m += 128) {
+        load(m + 64, b0, b1);
+        if (pend > 64) {
+          WaveSync();
+          if (!drain(63)) return false;
+        }
+        round(m, a0, a1);
+        if (m + 64 >= count) break;
+        load(m + 128, a0, a1);
+        round(m + 64, b0, b1);
       }
+      WaveSync();
     }
     // next iteration's tasks: the first spawn stays, idle lanes take queued
     // ones in lane order; their records load during any primitive tests
@@ -1791,11 +1801,14 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) Co
   uint32_t listedRays = 0;  // rays this wave finished by interval classification
   uint32_t stripe = blockIdx.x % kQueueStripes;  // home stripe; then any stripe with rays left
   // The next slot of the wave's stripe is claimed while the current ray is
-  // traced (its atomic's latency hidden), `ahead` until it is used.
+  // traced (its atomic's latency hidden), `ahead` until it is used, and its
+  // heavy-list entry read while the current ray is shaded (`entry`).
   uint32_t ahead = kNone;
+  uint2 entry = make_uint2(0u, 0u);
   for (;;) {
     __builtin_amdgcn_wave_barrier();
     uint32_t j = ahead;
+    const bool read = ahead != kNone;
     if (ahead == kNone) {
       j = 0;
       if (lane == 0) j = atomicAdd(heavyCtr + stripe * kStripeWords, 1u);
@@ -1817,7 +1830,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) Co
     }
     uint32_t next = 0;
     if (lane == 0) next = atomicAdd(heavyCtr + stripe * kStripeWords, 1u);
-    const uint2 e = p.heavy[k];
+    const uint2 e = read ? entry : p.heavy[k];
     const uint32_t jobWord = __builtin_amdgcn_readfirstlane(e.x);
     const Job& jb = kBatch ? p.jobs[jobWord & ~kResumeFlag] : p.job;
     const SceneView sv{jb.sc.ops, jb.sc.primA, jb.sc.primB, jb.sc.ops, 0u};
@@ -1861,8 +1874,10 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) Co
       int32_t* c = jb.counters + 6 * static_cast<size_t>(out);
       for (int i = 0; i < 6; i++) c[i] = v[i];
     }
-    ShadeAndStore<kFast>(jb, sv, r, h, out);
     ahead = __builtin_amdgcn_readfirstlane(next);
+    const uint32_t kn = stripe + kQueueStripes * ahead;
+    if (kn < count) entry = p.heavy[kn];
+    ShadeAndStore<kFast>(jb, sv, r, h, out);
   }
   // The last wave out adds the launch's heavy rays to the stream's running
   // total (words 6-7) and returns the heavy-ray counters to zero; each wave
