@@ -216,6 +216,9 @@ void PrepareCoop(const LaunchConfig& cfg, int32_t maxOps, int32_t maxLeaves, boo
 
 using namespace csgray::rt;
 
+static_assert(csgray::kGroupFlag == csgray::kWalkGroupFlag && csgray::kGroupChunk == csgray::kWalkChunk,
+              "the walk records' layout is shared by the scene build and the kernels");
+
 extern "C" {
 
 const char* csg_version(void) { return "csgray 0.1 (gfx950)"; }

@@ -183,7 +183,7 @@ size_t CoopLdsBytes(int32_t ops, int32_t leaves, bool lists, uint32_t* queueCap)
 // 16-byte state per operator, then the task ring's float tmins) and, when
 // `lists`, interval classification's (per operator two 4-byte child masks,
 // a 4-byte word and a 2-byte entry slot; then the crossings, 12 bytes each,
-// and the primitive tests of one iteration, 8 bytes each);
+// the pending primitive tests, 8 bytes each, and a round's hit chunks);
 // the ring's 16-bit tasks end the region in both.
 #if defined(__HIPCC__)
 #define CSG_HOST_DEVICE __host__ __device__
@@ -194,6 +194,12 @@ constexpr size_t kCoopOpStateBytes = 16;
 constexpr size_t kCoopTaskBytes = 6;
 constexpr uint32_t kCoopMaxCross = 31;  // crossings a ray's interval masks (32 bits) can order
 constexpr size_t kCoopJobs = 192;       // pending primitive tests: fewer than 64 left over + two a lane
+// Union groups in the interval walk's records (scene.hpp): the type-word flag
+// of a group's record, the members one chunk box covers, and the hit chunks
+// of one round of chunk tests (kept in LDS, 16 bits each).
+constexpr uint32_t kWalkGroupFlag = 0x200u;
+constexpr uint32_t kWalkChunk = 8;
+constexpr size_t kCoopHitChunks = 64;
 // Interval classification's crossings start after the per-operator state
 // (14 B an operator), 8-byte aligned for the primitive tests that follow them.
 CSG_HOST_DEVICE inline size_t CoopCrossOffset(int32_t ops) {
@@ -203,7 +209,7 @@ CSG_HOST_DEVICE inline size_t CoopWaveLds(int32_t ops, uint32_t queueCap, int32_
   const size_t machine =
       static_cast<size_t>(ops) * kCoopOpStateBytes + static_cast<size_t>(queueCap) * kCoopTaskBytes;
   const size_t masks = CoopCrossOffset(ops) + 12 * (static_cast<size_t>(kCoopMaxCross) + 1) + 8 * kCoopJobs +
-                       static_cast<size_t>(queueCap) * 2;
+                       2 * kCoopHitChunks + static_cast<size_t>(queueCap) * 2;
   const size_t bytes = lists && masks > machine ? masks : machine;
   return (bytes + 15) & ~static_cast<size_t>(15);
 }

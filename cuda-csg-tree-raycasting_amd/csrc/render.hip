@@ -1388,11 +1388,13 @@ __device__ Hit CoopTrace(const SceneView& sv, const RayF& r, OpState* st, uint16
 //      tmin 0 and again at the first crossing, so times and hit bits are the
 //      state machine's.  A Union group (a Union subtree flattened by the host
 //      into its members, the nodes under its Unions that are not Unions) is
-//      one ENTER: the whole wave tests its members' boxes, 64 a round.  A
-//      Union's solid is the union of its members' however the Unions pair
-//      them, and no member's flip parity changes inside the group, so the
-//      group's mask is the or of its members' masks: one walk iteration for
-//      the group where the tree takes one a level;
+//      one ENTER: the whole wave tests the boxes of its members' chunks (the
+//      host orders members spatially and boxes each 8), then the boxes of
+//      the members of the hit chunks, 64 a round.  A Union's solid is the
+//      union of its members' however the Unions pair them, and no member's
+//      flip parity changes inside the group, so the group's mask is the or
+//      of its members' masks: one walk iteration for the group where the
+//      tree takes one a level;
 //   B. the wave sorts the k <= 31 crossings by time (one per lane, each placed by a count);
 //      interval i of the ray is the stretch before crossing i (i = k: after
 //      the last), each crossed primitive's membership over the intervals
@@ -1428,13 +1430,13 @@ __device__ Hit CoopTrace(const SceneView& sv, const RayF& r, OpState* st, uint16
 // crossings (time, hit bits, tag: owner = operator << 2 | side << 1 | flip
 // parity, the crossed primitive's number << 24, 1 << 29 for an Exit alone),
 // the pending primitive tests (owner, descriptor; in B the crossed
-// primitives' masks), and the task ring (ENTER tasks: operator | flip parity
-// << 15) at the region's end.
+// primitives' masks), one round's hit chunks of a Union group (16 bits
+// each), and the task ring (ENTER tasks: operator | flip parity << 15) at the
+// region's end.
 constexpr uint32_t kOpUnion = 0, kOpDifference = 1;  // operator numbering (CSGTree.cuh:38-46)
 constexpr uint32_t kIvFlip = kFlip | kExit | kEnter;
 constexpr uint32_t kIvRoot = 0x7FFFu, kIvStampShift = 18;
 constexpr uint32_t kIvMaxIters = 1u << 14;  // walk iterations the 14-bit stamps number
-constexpr uint32_t kIvGroup = 0x200u;       // type word bit of a Union group's record (scene.hpp: kGroupFlag)
 constexpr uint32_t kIvPrimShift = 24, kIvLone = 1u << 29;
 
 // A hit primitive's crossings beyond 0, as the reference's intersection
@@ -1489,6 +1491,7 @@ __device__ bool IntervalTrace(const SceneView& sv, const RayF& r, void* region, 
   uint32_t* cb = reinterpret_cast<uint32_t*>(ct + kCoopMaxCross + 1);
   uint32_t* co = cb + kCoopMaxCross + 1;
   uint2* jobs = reinterpret_cast<uint2*>(co + kCoopMaxCross + 1);  // kCoopJobs pending primitive tests
+  uint16_t* hc = reinterpret_cast<uint16_t*>(jobs + kCoopJobs);     // kCoopHitChunks hit chunks
   uint32_t nCross = 0, nEntered = 0, pend = 0, nPrims = 0;  // wave-uniform
   // Tests the pending primitive tests from the first, 64 a batch, while more
   // than `keep` are pending, appends their crossings (in any order: B sorts
@@ -1565,7 +1568,7 @@ __device__ bool IntervalTrace(const SceneView& sv, const RayF& r, void* region, 
       *reinterpret_cast<uint2*>(msk + 2 * X) = make_uint2(0u, 0u);
       meta[X] = (parent == kNone ? kIvRoot : parent) | (op << 15) | (((typeWord >> 8) & 1u) << 17) |
                 (iter << kIvStampShift);
-      if (typeWord & kIvGroup) {
+      if (typeWord & kWalkGroupFlag) {
         grp = mine;  // a Union group: the wave tests its members below
       } else {
         const bool gL = BoxHit<true>(r, q0.x, q0.y, q0.z, q1.x, q1.y, q1.z, 0.0f);
@@ -1602,28 +1605,29 @@ __device__ bool IntervalTrace(const SceneView& sv, const RayF& r, void* region, 
     if (jobR != kNone) jobs[pend + nl + LanesBelow(bR)] = make_uint2(jobR, dR);
     pend += nl + static_cast<uint32_t>(__popcll(bR));
     WaveSync();
-    // the Union groups entered: members' boxes 64 a round, two register sets
-    // in turn (one round's records load during the other's tests); hit
+    // the Union groups entered: the boxes of their chunks (kWalkChunk members
+    // each), 64 a round, the hit ones listed in LDS; then the members of the
+    // hit chunks, 64 (8 chunks) a round, two rounds' records loading at once
+    // (unconditional loads: lanes without a member read the group's last
+    // one and ignore it, so the compiler counts the loads exactly).  Hit
     // primitives join the pending tests as the group's left child, hit
-    // operators are queued (at most 64 tests are pending before each pair of
-    // rounds: room for both)
+    // operators are queued (at most 64 tests are pending before each pair
+    // of rounds: room for both).
     for (uint64_t bg = __ballot(grp != kNone); bg != 0; bg &= bg - 1) {
       const int gl = __ffsll(static_cast<unsigned long long>(bg)) - 1;
       const uint32_t g = __builtin_amdgcn_readlane(grp, gl);
       const uint32_t first = __builtin_amdgcn_readlane(AsU(q3.x), gl);
       const uint32_t count = __builtin_amdgcn_readlane(AsU(q3.y), gl);
       const uint32_t owner = ((g & 0x7FFFu) << 2) | (g >> 15);
-      // unconditional loads (lanes past the last member read its record and
-      // ignore it), so that the compiler counts them exactly and waits only
-      // for the round it tests
-      auto load = [&](uint32_t m, float4& a, float4& b) {
-        const size_t at = first + 2 * static_cast<size_t>(min(m + static_cast<uint32_t>(lane), count - 1));
+      const uint32_t nChunks = (count + kWalkChunk - 1) / kWalkChunk;
+      const size_t mfirst = first + 2 * static_cast<size_t>(nChunks);
+      auto load = [&](size_t at, float4& a, float4& b) {
         a = sv.ops[at];
         b = sv.ops[at + 1];
       };
-      auto round = [&](uint32_t m, const float4& m0, const float4& m1) {
+      auto round = [&](bool valid, const float4& m0, const float4& m1) {
         uint32_t job = kNone, d = 0, spawn = kNone;
-        if (m + lane < count) {
+        if (valid) {
           const uint32_t md = AsU(m1.z);
           if (BoxHit<true>(r, m0.x, m0.y, m0.z, m0.w, m1.x, m1.y, 0.0f)) {
             if (DescType(md) < kSphereT) {
@@ -1641,20 +1645,36 @@ __device__ bool IntervalTrace(const SceneView& sv, const RayF& r, void* region, 
         if (job != kNone) jobs[pend + LanesBelow(bj)] = make_uint2(job, d);
         pend += static_cast<uint32_t>(__popcll(bj));
       };
-      float4 a0 = make_float4(0, 0, 0, 0), a1 = a0, b0 = a0, b1 = a0;
-      load(0, a0, a1);
-      for (uint32_t m = 0; m < count; m += 128) {
-        load(m + 64, b0, b1);
-        if (pend > 64) {
-          WaveSync();
-          if (!drain(63)) return false;
+      for (uint32_t c = 0; c < nChunks; c += 64) {
+        float4 c0, c1;
+        load(first + 2 * static_cast<size_t>(min(c + static_cast<uint32_t>(lane), nChunks - 1)), c0, c1);
+        const bool hit =
+            c + lane < nChunks && BoxHit<true>(r, c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, 0.0f);
+        const uint64_t hm = __ballot(hit);
+        if (hit) hc[LanesBelow(hm)] = static_cast<uint16_t>(c + lane);
+        const uint32_t nHit = static_cast<uint32_t>(__popcll(hm));
+        WaveSync();
+        // lane l takes member l % 8 of hit chunk h + l / 8 (count: none)
+        auto member = [&](uint32_t h) -> uint32_t {
+          const uint32_t k = h + (static_cast<uint32_t>(lane) >> 3);
+          const uint32_t mi = hc[min(k, nHit - 1)] * kWalkChunk + (static_cast<uint32_t>(lane) & 7u);
+          return k < nHit && mi < count ? mi : count;
+        };
+        for (uint32_t h = 0; h < nHit; h += 16) {
+          const uint32_t ia = member(h), ib = member(h + 8);
+          float4 a0, a1, b0, b1;
+          load(mfirst + 2 * static_cast<size_t>(min(ia, count - 1)), a0, a1);
+          load(mfirst + 2 * static_cast<size_t>(min(ib, count - 1)), b0, b1);
+          if (pend > 64) {
+            WaveSync();
+            if (!drain(63)) return false;
+          }
+          round(ia < count, a0, a1);
+          if (h + 8 >= nHit) break;
+          round(ib < count, b0, b1);
         }
-        round(m, a0, a1);
-        if (m + 64 >= count) break;
-        load(m + 128, a0, a1);
-        round(m + 64, b0, b1);
+        WaveSync();
       }
-      WaveSync();
     }
     // next iteration's tasks: the first spawn stays, idle lanes take queued
     // ones in lane order; their records load during any primitive tests

@@ -411,8 +411,39 @@ FlatScene Flatten(const CSGTree& tree) {
         continue;
       }
       if (members.size() < 3) continue;
+      // Morton order of the members' box centres within the group's box
+      // (10 bits an axis), then a box around each chunk of kGroupChunk
+      const Box& gb = tight[g];
+      std::vector<std::pair<uint32_t, int32_t>> keyed;
+      keyed.reserve(members.size());
+      for (const int32_t m : members) {
+        uint32_t code = 0;
+        for (int a = 0; a < 3; a++) {
+          const double span = static_cast<double>(gb.hi[a]) - gb.lo[a];
+          const double mid = 0.5 * (static_cast<double>(tight[m].lo[a]) + tight[m].hi[a]);
+          double u = span > 0.0 ? (mid - gb.lo[a]) / span : 0.0;
+          u = std::isfinite(u) ? std::min(1.0, std::max(0.0, u)) : 0.0;
+          const uint32_t qv = std::min<uint32_t>(1023u, static_cast<uint32_t>(u * 1024.0));
+          for (int bit = 0; bit < 10; bit++) code |= ((qv >> bit) & 1u) << (3 * bit + a);
+        }
+        keyed.emplace_back(code, m);
+      }
+      std::stable_sort(keyed.begin(), keyed.end(),
+                       [](const auto& x, const auto& y) { return x.first < y.first; });
+      for (size_t i = 0; i < keyed.size(); i++) members[i] = keyed[i].second;
       const uint32_t gi = static_cast<uint32_t>(opIndex[g]);
       const size_t first = f.walkOps.size();
+      for (size_t c0 = 0; c0 < members.size(); c0 += kGroupChunk) {
+        Box cb = tight[members[c0]];
+        for (size_t i = c0 + 1; i < std::min(members.size(), c0 + kGroupChunk); i++) {
+          for (int a = 0; a < 3; a++) {
+            cb.lo[a] = std::min(cb.lo[a], tight[members[i]].lo[a]);
+            cb.hi[a] = std::max(cb.hi[a], tight[members[i]].hi[a]);
+          }
+        }
+        f.walkOps.push_back(F4{cb.lo[0], cb.lo[1], cb.lo[2], cb.hi[0]});
+        f.walkOps.push_back(F4{cb.hi[1], cb.hi[2], 0.0f, 0.0f});
+      }
       for (const int32_t m : members) {
         const Box& b = tight[m];
         f.walkOps.push_back(F4{b.lo[0], b.lo[1], b.lo[2], b.hi[0]});

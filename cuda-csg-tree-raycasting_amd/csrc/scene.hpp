@@ -82,13 +82,17 @@ struct CSGTree {
 // mode's culling boxes, except that each Union group -- a Union whose parent
 // is not a Union, with the Unions below it, of 3 to kMaxGroupMembers members
 // (the nodes under those Unions that are not Unions) -- is flattened: the
-// group's record holds bits(first member record, as a float4 index into
-// walkOps), bits(member count), type | right<<8 | kGroupFlag, parent; each
-// member is two float4s after the operator records,
+// group's record holds bits(first chunk record, as a float4 index into
+// walkOps), bits(member count), type | right<<8 | kGroupFlag, parent.  After
+// the operator records, each group has one chunk record per kGroupChunk
+// members, (lo.x, lo.y, lo.z, hi.x), (hi.y, hi.z, 0, 0), the box around
+// those members, and then one record per member,
 //   (lo.x, lo.y, lo.z, hi.x), (hi.y, hi.z, bits(desc), 0),
-// with its culling box; a member that is an operator names the group as its
-// parent (as a left child).  A larger Union subtree stays binary at the top
-// and its Union children are tried as groups.
+// with its culling box, the members in Morton order of their boxes' centres
+// (so that a chunk's members lie close together); a member that is an
+// operator names the group as its parent (as a left child).  A larger Union
+// subtree stays binary at the top and its Union children are tried as
+// groups.
 //   primA[p] = (centre.xyz, radius or cube edge)
 //   primB[p] = (axis.xyz, height)
 //   primC[p] = (r, g, b, 0)
@@ -114,6 +118,7 @@ struct FlatScene {
 
 constexpr uint32_t kGroupFlag = 0x200u;    // type word bit of a flattened Union group (walkOps)
 constexpr int32_t kMaxGroupMembers = 1024;  // a group's members are tested 64 a round by one wave
+constexpr int32_t kGroupChunk = 8;          // members under one chunk box
 
 constexpr int32_t kMaxNodes = 65535;       // 16-bit node links
 constexpr int32_t kMaxPrimitives = 32767;  // the reference stores hit ids as int16 (Ray.cuh:41)
