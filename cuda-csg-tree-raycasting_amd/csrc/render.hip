@@ -1787,6 +1787,21 @@ __device__ bool IntervalTrace(const SceneView& sv, const RayF& r, void* region, 
   return true;
 }
 
+// The primary ray of output entry `out` (pixel * spp + sample) of a job.
+template <bool kFast>
+__device__ __forceinline__ RayF RayOfOutput(const Job& jb, uint32_t out) {
+  const uint32_t spp = static_cast<uint32_t>(jb.spp);
+  const uint32_t pixel = out / spp, sample = out - pixel * spp;
+  const int x = static_cast<int>(pixel % static_cast<uint32_t>(jb.width));
+  const int y = static_cast<int>(pixel / static_cast<uint32_t>(jb.width));
+  float ox = 0.5f, oy = 0.5f;
+  if (jb.spp > 1) {
+    ox = JitterUnit(JitterHash(jb.seed, pixel, sample, 0));
+    oy = JitterUnit(JitterHash(jb.seed, pixel, sample, 1));
+  }
+  return MakeRay<kFast>(jb, x, y, ox, oy);
+}
+
 // One wave per workgroup evaluates heavy rays one after another, reading
 // the scene from global memory (L2-resident): per-wave LDS holds only the
 // operator states and the task ring, so a CU keeps up to 16 rays in flight
@@ -1825,6 +1840,23 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) Co
   // heavy-list entry read while the current ray is shaded (`entry`).
   uint32_t ahead = kNone;
   uint2 entry = make_uint2(0u, 0u);
+  // Finished rays wait in `done` and are shaded kCoopShadeBatch at a time,
+  // one a lane (shading one ray with the whole wave would leave 63 lanes
+  // idle through its loads and arithmetic).
+  uint4* done = reinterpret_cast<uint4*>(reinterpret_cast<char*>(ldsBase) + regionBytes);
+  uint32_t nDone = 0;  // wave-uniform
+  auto shadeDone = [&]() {
+    WaveSync();
+    if (static_cast<uint32_t>(lane) < nDone) {
+      const uint4 d = done[lane];
+      const uint2 e = p.heavy[d.x];
+      const Job& jd = kBatch ? p.jobs[e.x & ~kResumeFlag] : p.job;
+      const SceneView sd{jd.sc.ops, jd.sc.primA, jd.sc.primB, jd.sc.ops, 0u};
+      ShadeAndStore<kFast>(jd, sd, RayOfOutput<kFast>(jd, e.y), Hit{__uint_as_float(d.y), d.z}, e.y);
+    }
+    WaveSync();
+    nDone = 0;
+  };
   for (;;) {
     __builtin_amdgcn_wave_barrier();
     uint32_t j = ahead;
@@ -1855,16 +1887,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) Co
     const Job& jb = kBatch ? p.jobs[jobWord & ~kResumeFlag] : p.job;
     const SceneView sv{jb.sc.ops, jb.sc.primA, jb.sc.primB, jb.sc.ops, 0u};
     const uint32_t out = __builtin_amdgcn_readfirstlane(e.y);
-    const uint32_t spp = static_cast<uint32_t>(jb.spp);
-    const uint32_t pixel = out / spp, sample = out - pixel * spp;
-    const int x = static_cast<int>(pixel % static_cast<uint32_t>(jb.width));
-    const int y = static_cast<int>(pixel / static_cast<uint32_t>(jb.width));
-    float ox = 0.5f, oy = 0.5f;
-    if (jb.spp > 1) {
-      ox = JitterUnit(JitterHash(jb.seed, pixel, sample, 0));
-      oy = JitterUnit(JitterHash(jb.seed, pixel, sample, 1));
-    }
-    const RayF r = MakeRay<kFast>(jb, x, y, ox, oy);
+    const RayF r = RayOfOutput<kFast>(jb, out);
     Counts n{0, 0, 0, 0, 0, 0};
     const uint32_t* rec = nullptr;
     const uint2* ent = nullptr;
@@ -1894,11 +1917,14 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) Co
       int32_t* c = jb.counters + 6 * static_cast<size_t>(out);
       for (int i = 0; i < 6; i++) c[i] = v[i];
     }
+    // the ray waits for its batch's shading (every lane stores the same bytes)
+    done[nDone] = make_uint4(k, __builtin_amdgcn_readfirstlane(AsU(h.t)), __builtin_amdgcn_readfirstlane(h.bits), 0u);
+    if (++nDone == kCoopShadeBatch) shadeDone();
     ahead = __builtin_amdgcn_readfirstlane(next);
     const uint32_t kn = stripe + kQueueStripes * ahead;
     if (kn < count) entry = p.heavy[kn];
-    ShadeAndStore<kFast>(jb, sv, r, h, out);
   }
+  if (nDone != 0) shadeDone();
   // The last wave out adds the launch's heavy rays to the stream's running
   // total (words 6-7) and returns the heavy-ray counters to zero; each wave
   // adds the rays it finished by interval classification to words 8-9.
@@ -1993,7 +2019,9 @@ const DeviceLimits* CurrentLimits() {
 
 int32_t MaxSupportedDepth() { return kMaxDepth; }
 
-size_t CoopLaunchLds(const RenderParams& p) { return CoopWaveLds(p.coopOps, p.coopQueueCap, p.coopLists); }
+size_t CoopLaunchLds(const RenderParams& p) {
+  return CoopWaveLds(p.coopOps, p.coopQueueCap, p.coopLists) + kCoopDoneBytes;
+}
 
 size_t CoopLdsBytes(int32_t ops, int32_t leaves, bool lists, uint32_t* queueCap) {
   if (ops > 32767) return 0;  // ring entries hold op << 1 | kind in 16 bits
@@ -2002,7 +2030,7 @@ size_t CoopLdsBytes(int32_t ops, int32_t leaves, bool lists, uint32_t* queueCap)
   uint32_t cap = 1;
   while (cap < static_cast<uint32_t>(live)) cap <<= 1;
   *queueCap = cap;
-  const size_t bytes = CoopWaveLds(ops, cap, lists ? 1 : 0);
+  const size_t bytes = CoopWaveLds(ops, cap, lists ? 1 : 0) + kCoopDoneBytes;
   const DeviceLimits* l = CurrentLimits();
   const size_t limit = l && l->ldsPerBlock ? l->ldsPerBlock : 64 * 1024;
   return bytes + kStaticLds <= limit ? bytes : 0;

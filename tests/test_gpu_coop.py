@@ -305,9 +305,10 @@ def test_plan_reports_cooperative_kernel(gpu):
     p = s.plan()
     print(f"\ncoop budget={p.coop_budget} grid={p.coop_grid} lds={p.coop_lds_bytes}")
     # one wave's LDS: 16 B per operator (512: 511 unions and the difference) and a 6-byte entry (task,
-    # tmin) per task-ring slot, as many as the operators without an operator child (256), 16-byte rounded
+    # tmin) per task-ring slot, as many as the operators without an operator child (256), 16-byte
+    # rounded, then 16 finished rays waiting for their shading (16 B each)
     assert p.coop_budget > 0 and p.coop_grid >= p.compute_units
-    assert p.coop_lds_bytes == (512 * 16 + 256 * 6 + 15) // 16 * 16
+    assert p.coop_lds_bytes == (512 * 16 + 256 * 6 + 15) // 16 * 16 + 16 * 16
     # 16 one-wave groups per CU fit
     assert p.coop_grid == 16 * p.compute_units
     assert s.plan(coop=-1).coop_budget == 0
@@ -347,7 +348,7 @@ def test_every_bottom_operator_live_at_once(gpu):
     sname, cam, light, phong, cam6, l3, ph8 = setups()[1]
     s = C.Scene(text)
     p = s.plan()
-    assert p.coop_lds_bytes == (1023 * 16 + 512 * 6 + 15) // 16 * 16
+    assert p.coop_lds_bytes == (1023 * 16 + 512 * 6 + 15) // 16 * 16 + 16 * 16
     w, h = 96, 54
     g = _np(s.render(cam, light, phong, w, h, mode=C.MODE_EXACT, counters=True, coop=1, coop_when=1))
     s.close()
