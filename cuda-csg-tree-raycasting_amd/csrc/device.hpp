@@ -202,9 +202,9 @@ constexpr uint32_t kWalkChunk = 8;
 constexpr size_t kCoopHitChunks = 64;
 // Rays the cooperative kernel has finished wait after the region (heavy slot,
 // hit time, hit bits; 16 B each) until kCoopShadeBatch of them are shaded at
-// once, one a lane.
-constexpr uint32_t kCoopShadeBatch = 16;
-constexpr size_t kCoopDoneBytes = 16 * kCoopShadeBatch;
+// once, one a lane; then 32 B of state for a ray classified in stretches.
+constexpr uint32_t kCoopShadeBatch = 14;
+constexpr size_t kCoopDoneBytes = 16 * (kCoopShadeBatch + 2);
 // Interval classification's crossings start after the per-operator state
 // (14 B an operator), 8-byte aligned for the primitive tests that follow them.
 CSG_HOST_DEVICE inline size_t CoopCrossOffset(int32_t ops) {

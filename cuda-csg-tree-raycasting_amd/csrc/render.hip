@@ -1438,6 +1438,11 @@ constexpr uint32_t kIvFlip = kFlip | kExit | kEnter;
 constexpr uint32_t kIvRoot = 0x7FFFu, kIvStampShift = 18;
 constexpr uint32_t kIvMaxIters = 1u << 14;  // walk iterations the 14-bit stamps number
 constexpr uint32_t kIvPrimShift = 24, kIvLone = 1u << 29;
+// IntervalTrace's outcomes (all nonzero): the stretch's first change of the
+// root's membership (res), no change in the stretch, more than 31 crossings
+// in the stretch, the ray goes to the state machine.
+constexpr int kIvHit = 1, kIvMiss = 2, kIvFull = 3, kIvFail = 4;
+constexpr int kIvMaxPasses = 24;  // walks a ray of more than 31 crossings may take in windows
 
 // A hit primitive's crossings beyond 0, as the reference's intersection
 // function reports them queried at tmin 0 and again at the first crossing
@@ -1480,10 +1485,12 @@ __device__ __forceinline__ uint32_t PrimCrossings(const SceneView& sv, const Ray
   return (Flags(b.bits) & kMissBit) ? 1u : 2u;
 }
 
-// Returns false when the ray must go to the state machine instead.  `sv.ops`
-// are the walk's records (DevScene::walkOps).
-__device__ bool IntervalTrace(const SceneView& sv, const RayF& r, void* region, int32_t ops, uint16_t* q,
-                              uint32_t qmask, int lane, Hit& res) {
+// Classifies the ray over its stretch [ts, te): only crossings in it are
+// kept, and each primitive enters the stretch in the state its earlier
+// crossings leave it in.  `sv.ops` are the walk's records
+// (DevScene::walkOps).
+__device__ int IntervalTrace(const SceneView& sv, const RayF& r, void* region, int32_t ops, uint16_t* q,
+                             uint32_t qmask, int lane, Hit& res, float ts, float te) {
   uint32_t* msk = reinterpret_cast<uint32_t*>(region);
   uint32_t* meta = msk + 2 * static_cast<size_t>(ops);
   uint16_t* entered = reinterpret_cast<uint16_t*>(meta + ops);
@@ -1494,35 +1501,44 @@ __device__ bool IntervalTrace(const SceneView& sv, const RayF& r, void* region, 
   uint16_t* hc = reinterpret_cast<uint16_t*>(jobs + kCoopJobs);     // kCoopHitChunks hit chunks
   uint32_t nCross = 0, nEntered = 0, pend = 0, nPrims = 0;  // wave-uniform
   // Tests the pending primitive tests from the first, 64 a batch, while more
-  // than `keep` are pending, appends their crossings (in any order: B sorts
-  // them), numbering each crossed primitive for its mask, and moves the
-  // untested rest (fewer than 64) to the front.  False: the ray goes to the
-  // state machine.
-  auto drain = [&](uint32_t keep) -> bool {
+  // than `keep` are pending, appends their crossings in the stretch (in any
+  // order: B sorts them), numbering each crossed primitive for its mask, and
+  // moves the untested rest (fewer than 64) to the front.  Returns 0, or the
+  // outcome that ends the walk (kIvFull, kIvFail).
+  auto drain = [&](uint32_t keep) -> int {
     uint32_t base = 0;
     bool bad = false;
     for (; base < pend && pend - base > keep; base += 64) {
-      uint32_t c = 0, owner = 0;
+      uint32_t owner = 0, n = 0;
+      bool lone = false;
       Hit a = MissRecord(), b = MissRecord();
       if (base + lane < pend) {
         const uint2 jb = jobs[base + lane];
         owner = jb.x;
-        c = PrimCrossings(sv, r, jb.y, a, b);
+        const uint32_t c = PrimCrossings(sv, r, jb.y, a, b);
         // (Enter, Exit) or (Exit) alone
         if (c == 2) bad |= !(a.bits & kEnter) || !(b.bits & kExit);
         else if (c == 1) bad |= !(a.bits & kExit);
+        const bool inA = c >= 1 && a.t >= ts && a.t < te;
+        const bool inB = c == 2 && b.t >= ts && b.t < te;
+        // inside through the whole stretch: no crossing in it, its mask all set
+        if (!inA && !inB && ((c == 1 && a.t >= te) || (c == 2 && a.t < ts && b.t >= te)))
+          atomicOr(&msk[2 * (owner >> 2) + ((owner >> 1) & 1u)], ~0u);
+        lone = inA ? c == 1 : inB;  // its first crossing in the stretch is an Exit: inside at ts
+        if (!inA) a = b;
+        n = (inA ? 1u : 0u) + (inB ? 1u : 0u);
       }
-      const uint64_t b1 = __ballot(c >= 1), b2 = __ballot(c == 2);
+      const uint64_t b1 = __ballot(n >= 1), b2 = __ballot(n == 2);
       const uint32_t total = static_cast<uint32_t>(__popcll(b1) + __popcll(b2));
-      if (nCross + total > kCoopMaxCross) return false;
-      const uint32_t tag = owner | ((nPrims + LanesBelow(b1)) << kIvPrimShift) | (c == 1 ? kIvLone : 0u);
+      if (nCross + total > kCoopMaxCross) return kIvFull;
+      const uint32_t tag = owner | ((nPrims + LanesBelow(b1)) << kIvPrimShift) | (lone ? kIvLone : 0u);
       const uint32_t at = nCross + LanesBelow(b1) + LanesBelow(b2);
-      if (c >= 1) {
+      if (n >= 1) {
         ct[at] = a.t;
         cb[at] = a.bits;
         co[at] = tag;
       }
-      if (c == 2) {
+      if (n == 2) {
         ct[at + 1] = b.t;
         cb[at + 1] = b.bits;
         co[at + 1] = tag;
@@ -1530,7 +1546,7 @@ __device__ bool IntervalTrace(const SceneView& sv, const RayF& r, void* region, 
       nCross += total;
       nPrims += static_cast<uint32_t>(__popcll(b1));
     }
-    if (__ballot(bad) != 0) return false;
+    if (__ballot(bad) != 0) return kIvFail;
     if (base != 0) {
       const uint32_t rest = base < pend ? pend - base : 0u;  // <= keep < 64
       uint2 v = make_uint2(0u, 0u);
@@ -1540,7 +1556,7 @@ __device__ bool IntervalTrace(const SceneView& sv, const RayF& r, void* region, 
       pend = rest;
     }
     WaveSync();
-    return true;
+    return 0;
   };
   // A: the walk down.  Each iteration enters the operators of the lanes'
   // tasks (box tests, spawns) and adds their hit primitive children to the
@@ -1667,7 +1683,7 @@ __device__ bool IntervalTrace(const SceneView& sv, const RayF& r, void* region, 
           load(mfirst + 2 * static_cast<size_t>(min(ib, count - 1)), b0, b1);
           if (pend > 64) {
             WaveSync();
-            if (!drain(63)) return false;
+            if (const int e = drain(63)) return e;
           }
           round(ia < count, a0, a1);
           if (h + 8 >= nHit) break;
@@ -1694,10 +1710,10 @@ __device__ bool IntervalTrace(const SceneView& sv, const RayF& r, void* region, 
     }
     // no task left (an idle lane would have taken any queued one): the walk is done
     const bool last = __ballot(mine != kNone) == 0;
-    if (!drain(last ? 0u : 63u)) return false;
+    if (const int e = drain(last ? 0u : 63u)) return e;
     if (last) break;
   }
-  if (iter >= kIvMaxIters) return false;  // stamps would overflow (not reached by trees of 32767 operators)
+  if (iter >= kIvMaxIters) return kIvFail;  // stamps would overflow (not reached by trees of 32767 operators)
   // B: sort the crossings, one a lane, and mark each primitive's intervals.
   // A lane's place is the number of crossings before its own in (time,
   // index) order, counted against each crossing broadcast in turn (k <= 31
@@ -1718,7 +1734,7 @@ __device__ bool IntervalTrace(const SceneView& sv, const RayF& r, void* region, 
     place += (ki < key || (ki == key && i < static_cast<uint32_t>(lane))) ? 1u : 0u;
     tie |= ki == key && i != static_cast<uint32_t>(lane);
   }
-  if (__ballot(tie && static_cast<uint32_t>(lane) < k) != 0) return false;
+  if (__ballot(tie && static_cast<uint32_t>(lane) < k) != 0) return kIvFail;
   if (static_cast<uint32_t>(lane) < k) {
     ct[place] = key;
     cb[place] = bits;
@@ -1772,19 +1788,19 @@ __device__ bool IntervalTrace(const SceneView& sv, const RayF& r, void* region, 
     WaveSync();
   }
   const uint64_t rootLane = __ballot(done);
-  if (rootLane == 0) return false;  // no root among the entered: a malformed tree
+  if (rootLane == 0) return kIvFail;  // no root among the entered: a malformed tree
   rootMask = __builtin_amdgcn_readlane(rootMask, __ffsll(static_cast<unsigned long long>(rootLane)) - 1);
   // the first crossing at which the root's membership changes
   const uint32_t change = (rootMask ^ (rootMask >> 1)) & ((1u << k) - 1u);
   if (change == 0) {
     res = MissRecord();
-    return true;
+    return kIvMiss;
   }
   const int j = __ffs(change) - 1;
   const uint32_t hb = bits ^ ((tag & 1u) ? kIvFlip : 0u);
   res.t = __uint_as_float(__builtin_amdgcn_readlane(AsU(key), j));
   res.bits = __builtin_amdgcn_readlane(hb, j);
-  return true;
+  return kIvHit;
 }
 
 // The primary ray of output entry `out` (pixel * spp + sample) of a job.
@@ -1844,6 +1860,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) Co
   // one a lane (shading one ray with the whole wave would leave 63 lanes
   // idle through its loads and arithmetic).
   uint4* done = reinterpret_cast<uint4*>(reinterpret_cast<char*>(ldsBase) + regionBytes);
+  uint4* stretch = done + kCoopShadeBatch;  // stretch state of a ray of more than 31 crossings: (ts, te), (w, answer)
   uint32_t nDone = 0;  // wave-uniform
   auto shadeDone = [&]() {
     WaveSync();
@@ -1905,7 +1922,68 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) Co
     if (kFast && !kCount && p.coopLists && jb.sc.conservative) {
       // the walk's records: boxes tight around each subtree's solid, Union groups flattened
       const SceneView tv{jb.sc.walkOps ? jb.sc.walkOps : jb.sc.ops, jb.sc.primA, jb.sc.primB, jb.sc.ops, 0u};
-      listed = IntervalTrace(tv, r, ldsBase, p.coopOps, q, qmask, lane, h);
+      // One walk over the whole ray.  A ray of more than 31 crossings is
+      // walked again over consecutive stretches of at most 31 crossings
+      // each (a quarter of its extent in the root's children's boxes to
+      // start with, halved while too many, then grown by half), all the way
+      // to its end: the first stretch in which the root's membership changes
+      // gives the answer, and a tie or a malformed crossing in any stretch
+      // sends the ray to the state machine, as the single walk would (two
+      // crossings of equal time always fall in one stretch).  The stretch
+      // state (bounds, width, answer so far) waits in LDS, so that the
+      // common single walk holds no more registers than before.
+      for (int pass = 0;; pass++) {
+        float ts = 0.0f, te = __builtin_inff();
+        if (pass > 0) {
+          const uint4 z = stretch[0];
+          ts = __uint_as_float(__builtin_amdgcn_readfirstlane(z.x));
+          te = __uint_as_float(__builtin_amdgcn_readfirstlane(z.y));
+        }
+        Hit hw = MissRecord();
+        const int st = IntervalTrace(tv, r, ldsBase, p.coopOps, q, qmask, lane, hw, ts, te);
+        if (pass == 0 && st != kIvFull) {
+          listed = st != kIvFail;
+          h = hw;
+          break;
+        }
+        if (st == kIvFail || pass == kIvMaxPasses) break;
+        auto uni = [](float x) { return __uint_as_float(__builtin_amdgcn_readfirstlane(AsU(x))); };
+        const float4 o0 = tv.ops[0], o1 = tv.ops[1], o2 = tv.ops[2];
+        float nL, xL, nR, xR;
+        Slab<true>(r, o0.x, o0.y, o0.z, o1.x, o1.y, o1.z, nL, xL);
+        Slab<true>(r, o0.w, o1.w, o2.x, o2.y, o2.z, o2.w, nR, xR);
+        const float t0 = uni(fmaxf(0.0f, fminf(nL, nR))), t1 = uni(fmaxf(xL, xR));
+        if (!(t1 > t0) || !(t1 < __builtin_inff())) break;
+        float w;
+        Hit ans;
+        if (pass == 0) {
+          w = uni(0.25f * (t1 - t0));
+          ans = MissRecord();
+        } else {
+          const uint4 z = stretch[1];
+          w = __uint_as_float(z.x);
+          ans = Hit{__uint_as_float(z.y), z.z};
+          if (st == kIvFull) {
+            w = uni(0.5f * w);
+          } else {
+            if (st == kIvHit && (Flags(ans.bits) & kMissBit)) ans = hw;
+            if (!(te < __builtin_inff())) {
+              listed = true;
+              h = ans;
+              break;
+            }
+            ts = te;
+            w = uni(1.5f * w);
+          }
+        }
+        te = uni((ts > 0.0f ? ts : t0) + w);
+        if (te >= t1) te = __builtin_inff();  // the last stretch reaches past every crossing
+        if (!(te > ts)) break;                // too short to make progress
+        stretch[0] = make_uint4(AsU(ts), AsU(te), 0u, 0u);
+        stretch[1] = make_uint4(AsU(w), __builtin_amdgcn_readfirstlane(AsU(ans.t)),
+                                __builtin_amdgcn_readfirstlane(ans.bits), 0u);
+        WaveSync();
+      }
     }
     if (listed) listedRays++;
     else h = CoopTrace<kFast, kCount>(sv, r, st, q, qt, qmask, table, lane, n, rec, ent, own);

@@ -167,7 +167,7 @@ def _union_of_operators(n=96, seed=11):
 
 
 @pytest.mark.parametrize("make,groups,share", [(lambda: _union_of_operators(96, seed=11), 13, 0.9),
-                                               (lambda: S.cheese(2048, seed=3), 2, 0.5)],
+                                               (lambda: S.cheese(2048, seed=3), 2, 0.9)],
                          ids=["union_of_operators96", "cheese2048_two_groups"])
 def test_interval_classification_across_union_groups(gpu, make, groups, share):
     """Union groups (flattened for the walk) whose members are operators
@@ -176,7 +176,7 @@ def test_interval_classification_across_union_groups(gpu, make, groups, share):
     classification, state machine and per-lane kernel agree bit for bit
     with every ray handed over.  (The 2048-sphere cheese is four times as
     dense as the benchmark's: about a third of its rays cross more than 31
-    surfaces and go to the state machine.)"""
+    surfaces and are classified over stretches of the ray.)"""
     print("\n[case] union groups fast, intervals vs state machine, 480x270", flush=True)
     sname, cam, light, phong, cam6, l3, ph8 = setups()[1]
     s = C.Scene(make())
