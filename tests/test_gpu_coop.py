@@ -189,6 +189,34 @@ def test_interval_classification_across_union_groups(gpu, make, groups, share):
     assert handed > 0 and classified > share * handed
 
 
+def test_batched_interval_classification_equals_per_lane(gpu):
+    """Fast mode, a batch of scenes whose primitives lie in their boxes (cheese
+    of 64, 512 and 2048 spheres -- the last with many rays of more than 31
+    crossings, classified over stretches -- and a Union of holed balls), every
+    ray handed over to the batched cooperative kernel: each job gets the bits
+    of its own per-lane render, and the handed-over rays are classified."""
+    print("\n[case] batch of interval-classified jobs, every ray handed over, fast", flush=True)
+    light, ph = C.light(S.LIGHT_TOWARD), C.phong(*S.PHONG)
+    sname, cam, light_, phong_, cam6, l3, ph8 = setups()[1]
+    texts = [S.cheese(64, seed=1), S.cheese(512, seed=512), S.cheese(2048, seed=3), _union_of_operators(48, seed=5)]
+    sizes = [(160, 120), (200, 150), (176, 99), (128, 128)]
+    specs, alone = [], []
+    for text, (w, h) in zip(texts, sizes):
+        sc = C.Scene(text)
+        alone.append(_np(sc.render(cam, light, ph, w, h, coop=-1)))
+        specs.append((sc, cam, light, ph, w, h, C.alloc_outputs(w, h)))
+    before, handed0 = C.coop_interval_rays(), C.coop_rays()
+    C.render_batch(C.make_jobs(specs), coop=1, coop_when=1)
+    torch.cuda.synchronize()
+    classified, handed = C.coop_interval_rays() - before, C.coop_rays() - handed0
+    print(f"handed={handed} classified={classified}")
+    for spec, a in zip(specs, alone):
+        _same(_np(spec[6]), a)
+    assert handed > 0 and classified > 0.99 * handed
+    for spec in specs:
+        spec[0].close()
+
+
 @pytest.mark.parametrize("eye,target", [((0.05, 0.02, 0.03), (1.0, 0.4, 0.2)), ((0.9, -0.7, 0.2), (-1.0, 0.3, -0.4))],
                          ids=["centre", "near_corner"])
 def test_interval_classification_from_inside_the_solid(gpu, eye, target):
