@@ -1803,6 +1803,16 @@ __device__ int IntervalTrace(const SceneView& sv, const RayF& r, void* region, i
   return kIvHit;
 }
 
+// Heavy-list slot of the j-th ray a stripe hands out: stripes take turns
+// in blocks of kHeavyBlock consecutive slots (stripe s: block s, s + 32, ...),
+// so that the rays a wave takes in a row, and shades together, are mostly
+// neighbours in the frame (a per-lane wave appends its rays together) and
+// their outputs share cache lines.
+constexpr uint32_t kHeavyBlock = 16;
+__device__ __forceinline__ uint64_t HeavySlot(uint32_t stripe, uint32_t j) {
+  return (uint64_t{j / kHeavyBlock} * kQueueStripes + stripe) * kHeavyBlock + j % kHeavyBlock;
+}
+
 // The primary ray of output entry `out` (pixel * spp + sample) of a job.
 template <bool kFast>
 __device__ __forceinline__ RayF RayOfOutput(const Job& jb, uint32_t out) {
@@ -1884,19 +1894,20 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) Co
       j = __builtin_amdgcn_readfirstlane(j);
     }
     ahead = kNone;
-    const uint32_t k = stripe + kQueueStripes * j;
-    if (k >= count) {
+    const uint64_t kSlot = HeavySlot(stripe, j);
+    if (kSlot >= count) {
       const uint32_t c = lane < static_cast<int>(kQueueStripes)
                              ? __hip_atomic_load(heavyCtr + lane * kStripeWords, __ATOMIC_RELAXED,
                                                  __HIP_MEMORY_SCOPE_AGENT)
                              : 0u;
       const bool left = lane < static_cast<int>(kQueueStripes) &&
-                        static_cast<uint64_t>(lane) + uint64_t{kQueueStripes} * c < static_cast<uint64_t>(count);
+                        HeavySlot(static_cast<uint32_t>(lane), c) < static_cast<uint64_t>(count);
       const uint64_t m = __ballot(left);
       if (m == 0) break;
       stripe = NextStripe(m, stripe);
       continue;
     }
+    const uint32_t k = static_cast<uint32_t>(kSlot);
     uint32_t next = 0;
     if (lane == 0) next = atomicAdd(heavyCtr + stripe * kStripeWords, 1u);
     const uint2 e = read ? entry : p.heavy[k];
@@ -1999,7 +2010,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) Co
     done[nDone] = make_uint4(k, __builtin_amdgcn_readfirstlane(AsU(h.t)), __builtin_amdgcn_readfirstlane(h.bits), 0u);
     if (++nDone == kCoopShadeBatch) shadeDone();
     ahead = __builtin_amdgcn_readfirstlane(next);
-    const uint32_t kn = stripe + kQueueStripes * ahead;
+    const uint64_t kn = HeavySlot(stripe, ahead);
     if (kn < count) entry = p.heavy[kn];
   }
   if (nDone != 0) shadeDone();
