@@ -1429,10 +1429,11 @@ __device__ Hit CoopTrace(const SceneView& sv, const RayF& r, OpState* st, uint16
 // entered it (18-31); then the entered operators in order (16 bits each), the
 // crossings (time, hit bits, tag: owner = operator << 2 | side << 1 | flip
 // parity, the crossed primitive's number << 24, 1 << 29 for an Exit alone),
-// the pending primitive tests (owner, descriptor; in B the crossed
-// primitives' masks), one round's hit chunks of a Union group (16 bits
-// each), and the task ring (ENTER tasks: operator | flip parity << 15) at the
-// region's end.
+// the pending primitive tests (owner, descriptor), one round's hit chunks of
+// a Union group (16 bits each), and the task ring (ENTER tasks: operator |
+// flip parity << 15) at the region's end.  A crossed primitive's number in
+// its crossings' tags tells its Exit (appended right after its Enter) from
+// a neighbour's crossing with the same owner.
 constexpr uint32_t kOpUnion = 0, kOpDifference = 1;  // operator numbering (CSGTree.cuh:38-46)
 constexpr uint32_t kIvFlip = kFlip | kExit | kEnter;
 constexpr uint32_t kIvRoot = 0x7FFFu, kIvStampShift = 18;
@@ -1714,11 +1715,12 @@ __device__ int IntervalTrace(const SceneView& sv, const RayF& r, void* region, i
     if (last) break;
   }
   if (iter >= kIvMaxIters) return kIvFail;  // stamps would overflow (not reached by trees of 32767 operators)
-  // B: sort the crossings, one a lane, and mark each primitive's intervals.
+  // B: rank the crossings, one a lane, and mark each primitive's intervals.
   // A lane's place is the number of crossings before its own in (time,
   // index) order, counted against each crossing broadcast in turn (k <= 31
   // scalar reads, no LDS round trips); any two of equal time send the ray
-  // to the state machine.
+  // to the state machine.  Interval i is the stretch before the crossing of
+  // place i; the crossings stay in their lanes.
   const uint32_t k = nCross;
   float key = __builtin_inff();
   uint32_t bits = 0u, tag = 0u;
@@ -1735,31 +1737,18 @@ __device__ int IntervalTrace(const SceneView& sv, const RayF& r, void* region, i
     tie |= ki == key && i != static_cast<uint32_t>(lane);
   }
   if (__ballot(tie && static_cast<uint32_t>(lane) < k) != 0) return kIvFail;
-  if (static_cast<uint32_t>(lane) < k) {
-    ct[place] = key;
-    cb[place] = bits;
-    co[place] = tag;
-  }
-  WaveSync();
-  if (static_cast<uint32_t>(lane) < k) {
-    key = ct[lane];
-    bits = cb[lane];
-    tag = co[lane];
-  }
-  // each crossed primitive's mask (the pending tests' space is free now):
-  // set after its Enter, clear after its Exit, all set up to an Exit alone;
-  // then or'ed into its owner's slot by the lane of its first crossing
+  // Each crossed primitive's mask, or'ed into its owner's slot by the lane
+  // of its first crossing: set after its Enter, up to its Exit -- appended
+  // right after the Enter, so it is the next lane's, with the same tag --,
+  // or all set up to an Exit alone.
   const bool held = static_cast<uint32_t>(lane) < k;
-  uint32_t* pm = reinterpret_cast<uint32_t*>(jobs);
-  const uint32_t pi = (tag >> kIvPrimShift) & 31u;
-  const bool firstX = held && ((bits & kEnter) || (tag & kIvLone));
-  if (firstX) pm[pi] = (bits & kEnter) ? 0u : ~0u;
-  WaveSync();
-  if (held && (bits & kEnter)) atomicOr(&pm[pi], ~0u << (lane + 1));
-  WaveSync();
-  if (held && !(bits & kEnter)) atomicAnd(&pm[pi], (2u << lane) - 1u);
-  WaveSync();
-  if (firstX) atomicOr(&msk[2 * ((tag >> 2) & 0x7FFFu) + ((tag >> 1) & 1u)], pm[pi]);
+  const bool enter = (bits & kEnter) != 0;
+  const uint32_t nextTag = __shfl_down(tag, 1), nextPlace = __shfl_down(place, 1);
+  if (held && (enter || (tag & kIvLone))) {
+    uint32_t m = enter ? (~0u << (place + 1)) : ((2u << place) - 1u);
+    if (enter && static_cast<uint32_t>(lane) + 1 < k && nextTag == tag) m &= (2u << nextPlace) - 1u;
+    atomicOr(&msk[2 * ((tag >> 2) & 0x7FFFu) + ((tag >> 1) & 1u)], m);
+  }
   WaveSync();
   // C: combine masks bottom-up, the operators entered in one iteration at a
   // time from the last iteration back (a child is always entered after its
@@ -1796,10 +1785,12 @@ __device__ int IntervalTrace(const SceneView& sv, const RayF& r, void* region, i
     res = MissRecord();
     return kIvMiss;
   }
-  const int j = __ffs(change) - 1;
+  // the crossing of place j: the lane that holds it
+  const uint32_t j = static_cast<uint32_t>(__ffs(change) - 1);
+  const int src = __ffsll(static_cast<unsigned long long>(__ballot(held && place == j))) - 1;
   const uint32_t hb = bits ^ ((tag & 1u) ? kIvFlip : 0u);
-  res.t = __uint_as_float(__builtin_amdgcn_readlane(AsU(key), j));
-  res.bits = __builtin_amdgcn_readlane(hb, j);
+  res.t = __uint_as_float(__builtin_amdgcn_readlane(AsU(key), src));
+  res.bits = __builtin_amdgcn_readlane(hb, src);
   return kIvHit;
 }
 
