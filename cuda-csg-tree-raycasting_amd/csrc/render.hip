@@ -943,6 +943,19 @@ __device__ __forceinline__ uint32_t NextStripe(uint64_t m, uint32_t stripe) {
   return static_cast<uint32_t>(__ffsll(static_cast<unsigned long long>(after != 0 ? after : m)) - 1);
 }
 
+// The launch's parameters, read afresh from the kernel-argument segment
+// (a kernel's one argument starts at its offset 0).  Through an opaque copy
+// of the segment's address the compiler cannot keep every field a loop
+// reads in scalar registers for the whole launch: the cooperative kernel's
+// exceeded the 102 a wave has, and the rest were parked in lanes of vector
+// registers, a vector instruction for each park and each read back (about
+// 300 of them in its loop, 30 now; the cheese frame 1.5 % faster).
+__device__ __forceinline__ const RenderParams& FreshParams() {
+  auto k = __builtin_amdgcn_kernarg_segment_ptr();
+  asm volatile("" : "+s"(k));
+  return *(const RenderParams*)k;
+}
+
 template <bool kFast, bool kCount, bool kLds, bool kBatch, bool kXcd, bool kCache = false>
 __device__ __forceinline__ void RenderBody(const RenderParams& p) {
   extern __shared__ uint4 ldsBase[];
@@ -991,6 +1004,7 @@ __device__ __forceinline__ void RenderBody(const RenderParams& p) {
       kXcd ? HomeStripe(blockIdx.x, (blockIdx.x / 8) * (blockDim.x >> 6) + (tid >> 6))
            : (blockIdx.x * (blockDim.x >> 6) + (tid >> 6)) % kQueueStripes);
   for (;;) {
+    const RenderParams& p = FreshParams();  // the loop reads `p` afresh each tile
     uint32_t j = 0;
     if (lane == 0) j = atomicAdd(tileCtr + stripe * kStripeWords, 1u);
     j = __builtin_amdgcn_readfirstlane(j);
@@ -1864,6 +1878,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) Co
   uint4* stretch = done + kCoopShadeBatch;  // stretch state of a ray of more than 31 crossings: (ts, te), (w, answer)
   uint32_t nDone = 0;  // wave-uniform
   auto shadeDone = [&]() {
+    const RenderParams& p = FreshParams();
     WaveSync();
     if (static_cast<uint32_t>(lane) < nDone) {
       const uint4 d = done[lane];
@@ -1876,6 +1891,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) Co
     nDone = 0;
   };
   for (;;) {
+    const RenderParams& p = FreshParams();  // the loop reads `p` afresh each ray
     __builtin_amdgcn_wave_barrier();
     uint32_t j = ahead;
     const bool read = ahead != kNone;
@@ -1901,7 +1917,10 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) Co
     const uint32_t k = static_cast<uint32_t>(kSlot);
     uint32_t next = 0;
     if (lane == 0) next = atomicAdd(heavyCtr + stripe * kStripeWords, 1u);
-    const uint2 e = read ? entry : p.heavy[k];
+    // (a choice between values: one between `entry`'s address and the
+    // list's would keep `entry` in scratch memory)
+    if (!read) entry = p.heavy[k];
+    const uint2 e = entry;
     const uint32_t jobWord = __builtin_amdgcn_readfirstlane(e.x);
     const Job& jb = kBatch ? p.jobs[jobWord & ~kResumeFlag] : p.job;
     const SceneView sv{jb.sc.ops, jb.sc.primA, jb.sc.primB, jb.sc.ops, 0u};
