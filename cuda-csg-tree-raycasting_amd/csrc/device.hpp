@@ -166,10 +166,17 @@ constexpr uint32_t kHeavyCap = 1u << 22;
 // action, stack slots, tmin, the L/R/held records, the event counts so far),
 // then per ray and stack slot the slot's entry (8 B) and the operator it
 // belongs to (4 B).  kResumeFlag in a heavy-list entry's job word marks a
-// ray with a record.
+// ray with a record.  Words kResumeDirWord-+2 of a slot's record hold the
+// ray's direction, whether or not the ray left its state (kDirFlag): the
+// cooperative kernel then starts from it instead of building the ray again.
 constexpr size_t kResumeBytes = size_t{64} << 20;
 constexpr uint32_t kResumeWords = 20;
+constexpr uint32_t kResumeDirWord = 16;
 constexpr uint32_t kResumeFlag = 1u << 31;
+constexpr uint32_t kDirFlag = 1u << 30;
+constexpr uint32_t kJobWordIndex = ~(kResumeFlag | kDirFlag);
+static_assert(kResumeDirWord + 4 <= kResumeWords && kResumeDirWord % 4 == 0 && kResumeWords % 4 == 0,
+              "the direction takes one aligned 16-byte word of the record");
 // Bytes of one heavy list together with its resume area.
 constexpr size_t kHeavyBytes = sizeof(uint32_t) * 2 * kHeavyCap + kResumeBytes;
 

@@ -694,6 +694,8 @@ __device__ __forceinline__ float SlabRcp(float d) {
 // reference's arithmetic).  Fast mode keeps the hardware reciprocals, which
 // only the box slabs and the cube's slab use.
 template <bool kFast>
+__device__ __forceinline__ RayF RayOfDirection(const Job& jb, float dx, float dy, float dz);
+template <bool kFast>
 __device__ __forceinline__ RayF MakeRay(const Job& jb, int x, int y, float ox, float oy) {
   const float width = static_cast<float>(jb.width), height = static_cast<float>(jb.height);
   const float u = (static_cast<float>(x) + ox) / (width - 1);
@@ -706,13 +708,20 @@ __device__ __forceinline__ RayF MakeRay(const Job& jb, int x, int y, float ox, f
   Normalize3<false>(dx, dy, dz);
   const float len2 = dx * dx + dy * dy + dz * dz;
   const float invLen = 1.0f / sqrtf(len2);
+  return RayOfDirection<kFast>(jb, dx * invLen, dy * invLen, dz * invLen);
+}
+
+// The ray from the job's camera along direction (dx, dy, dz), as MakeRay
+// finishes it.
+template <bool kFast>
+__device__ __forceinline__ RayF RayOfDirection(const Job& jb, float dx, float dy, float dz) {
   RayF r;
   r.ox = jb.pos[0];
   r.oy = jb.pos[1];
   r.oz = jb.pos[2];
-  r.dx = dx * invLen;
-  r.dy = dy * invLen;
-  r.dz = dz * invLen;
+  r.dx = dx;
+  r.dy = dy;
+  r.dz = dz;
   r.ix = kFast ? SlabRcp(r.dx) : 1.0f / r.dx;
   r.iy = kFast ? SlabRcp(r.dy) : 1.0f / r.dy;
   r.iz = kFast ? SlabRcp(r.dz) : 1.0f / r.dz;
@@ -880,9 +889,12 @@ __device__ __forceinline__ void RenderTile(const Job& jb, const SceneView& sv, u
       const uint32_t rcap = ResumeCap(p, cap);
       const bool classify = kFast && !kCount && p.coopLists && jb.sc.conservative;
       const bool keep = !classify && slot < rcap && ts.slots + 2 <= 64 && ts.cur != kNone;
-      p.heavy[slot] = make_uint2(jobIndex | (keep ? kResumeFlag : 0u), static_cast<uint32_t>(out));
+      const bool dir = slot < rcap;  // the ray's direction goes with it
+      p.heavy[slot] = make_uint2(jobIndex | (keep ? kResumeFlag : 0u) | (dir ? kDirFlag : 0u),
+                                 static_cast<uint32_t>(out));
+      uint32_t* rec = p.resume + static_cast<size_t>(slot) * kResumeWords;
+      if (dir) *reinterpret_cast<uint4*>(rec + kResumeDirWord) = make_uint4(AsU(r.dx), AsU(r.dy), AsU(r.dz), 0u);
       if (keep) {
-        uint32_t* rec = p.resume + static_cast<size_t>(slot) * kResumeWords;
         rec[0] = ts.cur;
         rec[1] = ts.action;
         rec[2] = static_cast<uint32_t>(ts.slots);
@@ -1871,6 +1883,13 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) Co
   // heavy-list entry read while the current ray is shaded (`entry`).
   uint32_t ahead = kNone;
   uint2 entry = make_uint2(0u, 0u);
+  // the direction the per-lane kernel left in the slot's resume record
+  // (slots below dirCap have one), read with the entry
+  const uint32_t dirCap = ResumeCap(p, p.stackSlots > 0 ? p.stackSlots : 1);
+  uint4 dirw = make_uint4(0u, 0u, 0u, 0u);
+  auto dirOf = [&](const RenderParams& pp, uint64_t slot) {
+    return *reinterpret_cast<const uint4*>(pp.resume + slot * kResumeWords + kResumeDirWord);
+  };
   // Finished rays wait in `done` and are shaded kCoopShadeBatch at a time,
   // one a lane (shading one ray with the whole wave would leave 63 lanes
   // idle through its loads and arithmetic).
@@ -1883,7 +1902,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) Co
     if (static_cast<uint32_t>(lane) < nDone) {
       const uint4 d = done[lane];
       const uint2 e = p.heavy[d.x];
-      const Job& jd = kBatch ? p.jobs[e.x & ~kResumeFlag] : p.job;
+      const Job& jd = kBatch ? p.jobs[e.x & kJobWordIndex] : p.job;
       const SceneView sd{jd.sc.ops, jd.sc.primA, jd.sc.primB, jd.sc.ops, 0u};
       ShadeAndStore<kFast>(jd, sd, RayOfOutput<kFast>(jd, e.y), Hit{__uint_as_float(d.y), d.z}, e.y);
     }
@@ -1919,13 +1938,20 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) Co
     if (lane == 0) next = atomicAdd(heavyCtr + stripe * kStripeWords, 1u);
     // (a choice between values: one between `entry`'s address and the
     // list's would keep `entry` in scratch memory)
-    if (!read) entry = p.heavy[k];
+    if (!read) {
+      entry = p.heavy[k];
+      if (k < dirCap) dirw = dirOf(p, k);
+    }
     const uint2 e = entry;
     const uint32_t jobWord = __builtin_amdgcn_readfirstlane(e.x);
-    const Job& jb = kBatch ? p.jobs[jobWord & ~kResumeFlag] : p.job;
+    const Job& jb = kBatch ? p.jobs[jobWord & kJobWordIndex] : p.job;
     const SceneView sv{jb.sc.ops, jb.sc.primA, jb.sc.primB, jb.sc.ops, 0u};
     const uint32_t out = __builtin_amdgcn_readfirstlane(e.y);
-    const RayF r = RayOfOutput<kFast>(jb, out);
+    // (the direction, when the ray brought it, is the one the ray would be
+    // built with: the same MakeRay made it)
+    auto uniF = [](uint32_t x) { return __uint_as_float(__builtin_amdgcn_readfirstlane(x)); };
+    const RayF r = (jobWord & kDirFlag) ? RayOfDirection<kFast>(jb, uniF(dirw.x), uniF(dirw.y), uniF(dirw.z))
+                                        : RayOfOutput<kFast>(jb, out);
     Counts n{0, 0, 0, 0, 0, 0};
     const uint32_t* rec = nullptr;
     const uint2* ent = nullptr;
@@ -2021,7 +2047,10 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) Co
     if (++nDone == kCoopShadeBatch) shadeDone();
     ahead = __builtin_amdgcn_readfirstlane(next);
     const uint64_t kn = HeavySlot(stripe, ahead);
-    if (kn < count) entry = p.heavy[kn];
+    if (kn < count) {
+      entry = p.heavy[kn];
+      if (kn < dirCap) dirw = dirOf(p, kn);
+    }
   }
   if (nDone != 0) shadeDone();
   // The last wave out adds the launch's heavy rays to the stream's running
