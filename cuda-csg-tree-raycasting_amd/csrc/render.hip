@@ -1742,32 +1742,34 @@ __device__ int IntervalTrace(const SceneView& sv, const RayF& r, void* region, i
   }
   if (iter >= kIvMaxIters) return kIvFail;  // stamps would overflow (not reached by trees of 32767 operators)
   // B: rank the crossings, one a lane, and mark each primitive's intervals.
-  // A lane's place is the number of crossings before its own in (time,
-  // index) order, counted against each crossing broadcast in turn (k <= 31
-  // scalar reads, no LDS round trips); any two of equal time send the ray
-  // to the state machine.  Interval i is the stretch before the crossing of
-  // place i; the crossings stay in their lanes.
+  // A lane's place is the number of crossings of earlier time, counted
+  // against each crossing broadcast in turn (k <= 31 scalar reads, no LDS
+  // round trips).  Crossings of equal time share a place: each crossing
+  // claims its place's word (the crossing times are in registers now), and
+  // a lane that finds another lane's claim there sends the ray to the state
+  // machine.  Interval i is the stretch before the crossing of place i; the
+  // crossings stay in their lanes.
   const uint32_t k = nCross;
+  const bool held = static_cast<uint32_t>(lane) < k;
   float key = __builtin_inff();
   uint32_t bits = 0u, tag = 0u;
-  if (static_cast<uint32_t>(lane) < k) {
+  if (held) {
     key = ct[lane];
     bits = cb[lane];
     tag = co[lane];
   }
   uint32_t place = 0;
-  bool tie = false;
-  for (uint32_t i = 0; i < k; i++) {
-    const float ki = __uint_as_float(__builtin_amdgcn_readlane(AsU(key), static_cast<int>(i)));
-    place += (ki < key || (ki == key && i < static_cast<uint32_t>(lane))) ? 1u : 0u;
-    tie |= ki == key && i != static_cast<uint32_t>(lane);
-  }
-  if (__ballot(tie && static_cast<uint32_t>(lane) < k) != 0) return kIvFail;
+  for (uint32_t i = 0; i < k; i++)
+    place += __uint_as_float(__builtin_amdgcn_readlane(AsU(key), static_cast<int>(i))) < key ? 1u : 0u;
+  uint32_t* claim = reinterpret_cast<uint32_t*>(ct);  // place < k <= 31 for a held crossing
+  WaveSync();
+  if (held) claim[place] = static_cast<uint32_t>(lane);
+  WaveSync();
+  if (__ballot(held && claim[place] != static_cast<uint32_t>(lane)) != 0) return kIvFail;
   // Each crossed primitive's mask, or'ed into its owner's slot by the lane
   // of its first crossing: set after its Enter, up to its Exit -- appended
   // right after the Enter, so it is the next lane's, with the same tag --,
   // or all set up to an Exit alone.
-  const bool held = static_cast<uint32_t>(lane) < k;
   const bool enter = (bits & kEnter) != 0;
   const uint32_t nextTag = __shfl_down(tag, 1), nextPlace = __shfl_down(place, 1);
   if (held && (enter || (tag & kIvLone))) {
@@ -1902,9 +1904,13 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) Co
     if (static_cast<uint32_t>(lane) < nDone) {
       const uint4 d = done[lane];
       const uint2 e = p.heavy[d.x];
+      const uint4 w = d.x < dirCap ? dirOf(p, d.x) : make_uint4(0u, 0u, 0u, 0u);
       const Job& jd = kBatch ? p.jobs[e.x & kJobWordIndex] : p.job;
       const SceneView sd{jd.sc.ops, jd.sc.primA, jd.sc.primB, jd.sc.ops, 0u};
-      ShadeAndStore<kFast>(jd, sd, RayOfOutput<kFast>(jd, e.y), Hit{__uint_as_float(d.y), d.z}, e.y);
+      const RayF rd = (e.x & kDirFlag) ? RayOfDirection<kFast>(jd, __uint_as_float(w.x), __uint_as_float(w.y),
+                                                               __uint_as_float(w.z))
+                                       : RayOfOutput<kFast>(jd, e.y);
+      ShadeAndStore<kFast>(jd, sd, rd, Hit{__uint_as_float(d.y), d.z}, e.y);
     }
     WaveSync();
     nDone = 0;

@@ -80,22 +80,34 @@ def test_cheese_split_between_kernels_matches_oracle(gpu):
 
 def test_fast_mode_same_with_and_without_deferral(gpu):
     """Fast mode, cheese 1920x1080: budgets 1, 64, default and never give the
-    same bits in all four outputs.  Budget 1 hands over every ray, more than
-    the resume area has records (64 MiB of 80 + 12 x 10 bytes for the
-    cheese's 10-slot stacks): the first rays bring their direction to the
-    cooperative kernel in their record, the rest are rebuilt from their
-    pixel, and both agree with the per-lane kernel."""
+    same bits in all four outputs."""
     print("\n[case] cheese 512 1920x1080 fast, budgets 1/64/default/never", flush=True)
     sname, cam, light, phong, cam6, l3, ph8 = setups()[1]
     s = C.Scene(S.cheese(512, seed=512))
     ref = _np(s.render(cam, light, phong, 1920, 1080, coop=-1))
-    records = (64 << 20) // (80 + 12 * 10)
     for budget, when in ((1, 1), (64, 1), (64, 0), (0, 0)):
-        before = C.coop_rays()
         _same(_np(s.render(cam, light, phong, 1920, 1080, coop=budget, coop_when=when)), ref)
-        if budget == 1:
-            assert C.coop_rays() - before > records
     s.close()
+
+
+def test_more_handed_over_rays_than_resume_records(gpu):
+    """Fast mode, cheese 3840x2160, budget 1: more rays are handed over than
+    the resume area has records (64 MiB of 80 + 12 x 10 bytes for the
+    cheese's 10-slot stacks).  The first rays bring their direction to the
+    cooperative kernel in their record, the rest are rebuilt from their
+    pixel; interval classification of both gives the per-lane kernel's bits."""
+    print("\n[case] cheese 512 3840x2160 fast, budget 1 eager vs per-lane", flush=True)
+    sname, cam, light, phong, cam6, l3, ph8 = setups()[1]
+    s = C.Scene(S.cheese(512, seed=512))
+    ref = _np(s.render(cam, light, phong, 3840, 2160, coop=-1))
+    before, classified0 = C.coop_rays(), C.coop_interval_rays()
+    got = _np(s.render(cam, light, phong, 3840, 2160, coop=1, coop_when=1))
+    handed, classified = C.coop_rays() - before, C.coop_interval_rays() - classified0
+    s.close()
+    print(f"rays handed over: {handed}, classified by intervals: {classified}")
+    assert handed > (64 << 20) // (80 + 12 * 10)
+    assert classified > 0.99 * handed
+    _same(got, ref)
 
 
 def _three_ways(s, cam, light, phong, w, h, **kw):
