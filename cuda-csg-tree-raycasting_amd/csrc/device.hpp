@@ -75,7 +75,8 @@ struct RenderParams {
   uint32_t coopWaves;         // waves in the cooperative grid
   // Resume area (`resumeBytes`, right after the heavy list): for the first
   // ResumeCap() handed-over rays, the state machine's state at hand-over, so
-  // the cooperative kernel continues the ray instead of restarting it.
+  // the cooperative kernel continues the ray instead of restarting it; its
+  // last eighth the first DirCap() handed-over rays' directions.
   uint32_t* resume;
   uint32_t resumeBytes;
   int32_t cacheOps;           // plain global-memory launches: operator records [0, cacheOps) staged in LDS
@@ -166,17 +167,17 @@ constexpr uint32_t kHeavyCap = 1u << 22;
 // action, stack slots, tmin, the L/R/held records, the event counts so far),
 // then per ray and stack slot the slot's entry (8 B) and the operator it
 // belongs to (4 B).  kResumeFlag in a heavy-list entry's job word marks a
-// ray with a record.  Words kResumeDirWord-+2 of a slot's record hold the
-// ray's direction, whether or not the ray left its state (kDirFlag): the
-// cooperative kernel then starts from it instead of building the ray again.
+// ray with a record.  The last eighth of the area holds the directions of
+// the first handed-over rays, 16 B a slot (kDirFlag marks a ray with one),
+// whether or not they left their state: the cooperative kernel starts from
+// it instead of building the ray again, and neighbouring slots share cache
+// lines.  The records take the rest.
 constexpr size_t kResumeBytes = size_t{64} << 20;
 constexpr uint32_t kResumeWords = 20;
-constexpr uint32_t kResumeDirWord = 16;
 constexpr uint32_t kResumeFlag = 1u << 31;
 constexpr uint32_t kDirFlag = 1u << 30;
 constexpr uint32_t kJobWordIndex = ~(kResumeFlag | kDirFlag);
-static_assert(kResumeDirWord + 4 <= kResumeWords && kResumeDirWord % 4 == 0 && kResumeWords % 4 == 0,
-              "the direction takes one aligned 16-byte word of the record");
+constexpr size_t ResumeDirBytes(size_t resumeBytes) { return (resumeBytes / 8) & ~size_t{15}; }
 // Bytes of one heavy list together with its resume area.
 constexpr size_t kHeavyBytes = sizeof(uint32_t) * 2 * kHeavyCap + kResumeBytes;
 

@@ -831,10 +831,18 @@ __device__ __forceinline__ uint32_t Compact1By1(uint32_t v) {
 
 // Resume area of a launch whose stacks hold `cap` slots: ResumeCap records
 // of kResumeWords words, then ResumeCap x cap stack entries, then as many
-// owners (the operator each stack slot belongs to).
+// owners (the operator each stack slot belongs to); its last eighth holds
+// the directions of the first DirCap handed-over rays.
 __device__ __forceinline__ uint32_t ResumeCap(const RenderParams& p, int cap) {
-  const size_t n = p.resumeBytes / (kResumeWords * 4 + 12 * static_cast<size_t>(cap));
+  const size_t n = (p.resumeBytes - ResumeDirBytes(p.resumeBytes)) / (kResumeWords * 4 + 12 * static_cast<size_t>(cap));
   return n < p.heavyCap ? static_cast<uint32_t>(n) : p.heavyCap;
+}
+__device__ __forceinline__ uint32_t DirCap(const RenderParams& p) {
+  const size_t n = ResumeDirBytes(p.resumeBytes) / sizeof(uint4);
+  return n < p.heavyCap ? static_cast<uint32_t>(n) : p.heavyCap;
+}
+__device__ __forceinline__ uint4* Directions(const RenderParams& p) {
+  return reinterpret_cast<uint4*>(reinterpret_cast<char*>(p.resume) + p.resumeBytes - ResumeDirBytes(p.resumeBytes));
 }
 __device__ __forceinline__ uint2* ResumeEntries(const RenderParams& p, uint32_t rcap) {
   return reinterpret_cast<uint2*>(p.resume + static_cast<size_t>(rcap) * kResumeWords);
@@ -889,12 +897,12 @@ __device__ __forceinline__ void RenderTile(const Job& jb, const SceneView& sv, u
       const uint32_t rcap = ResumeCap(p, cap);
       const bool classify = kFast && !kCount && p.coopLists && jb.sc.conservative;
       const bool keep = !classify && slot < rcap && ts.slots + 2 <= 64 && ts.cur != kNone;
-      const bool dir = slot < rcap;  // the ray's direction goes with it
+      const bool dir = slot < DirCap(p);  // the ray's direction goes with it
       p.heavy[slot] = make_uint2(jobIndex | (keep ? kResumeFlag : 0u) | (dir ? kDirFlag : 0u),
                                  static_cast<uint32_t>(out));
-      uint32_t* rec = p.resume + static_cast<size_t>(slot) * kResumeWords;
-      if (dir) *reinterpret_cast<uint4*>(rec + kResumeDirWord) = make_uint4(AsU(r.dx), AsU(r.dy), AsU(r.dz), 0u);
+      if (dir) Directions(p)[slot] = make_uint4(AsU(r.dx), AsU(r.dy), AsU(r.dz), 0u);
       if (keep) {
+        uint32_t* rec = p.resume + static_cast<size_t>(slot) * kResumeWords;
         rec[0] = ts.cur;
         rec[1] = ts.action;
         rec[2] = static_cast<uint32_t>(ts.slots);
@@ -1887,10 +1895,10 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) Co
   uint2 entry = make_uint2(0u, 0u);
   // the direction the per-lane kernel left in the slot's resume record
   // (slots below dirCap have one), read with the entry
-  const uint32_t dirCap = ResumeCap(p, p.stackSlots > 0 ? p.stackSlots : 1);
+  const uint32_t dirCap = DirCap(p);
   uint4 dirw = make_uint4(0u, 0u, 0u, 0u);
   auto dirOf = [&](const RenderParams& pp, uint64_t slot) {
-    return *reinterpret_cast<const uint4*>(pp.resume + slot * kResumeWords + kResumeDirWord);
+    return Directions(pp)[slot];
   };
   // Finished rays wait in `done` and are shaded kCoopShadeBatch at a time,
   // one a lane (shading one ray with the whole wave would leave 63 lanes

@@ -90,12 +90,12 @@ def test_fast_mode_same_with_and_without_deferral(gpu):
     s.close()
 
 
-def test_more_handed_over_rays_than_resume_records(gpu):
+def test_more_handed_over_rays_than_direction_slots(gpu):
     """Fast mode, cheese 3840x2160, budget 1: more rays are handed over than
-    the resume area has records (64 MiB of 80 + 12 x 10 bytes for the
-    cheese's 10-slot stacks).  The first rays bring their direction to the
-    cooperative kernel in their record, the rest are rebuilt from their
-    pixel; interval classification of both gives the per-lane kernel's bits."""
+    the resume area keeps directions for (its last eighth, 16 B a ray: 512 Ki
+    rays).  The first rays bring their direction to the cooperative kernel,
+    the rest are rebuilt from their pixel; interval classification of both
+    gives the per-lane kernel's bits."""
     print("\n[case] cheese 512 3840x2160 fast, budget 1 eager vs per-lane", flush=True)
     sname, cam, light, phong, cam6, l3, ph8 = setups()[1]
     s = C.Scene(S.cheese(512, seed=512))
@@ -105,7 +105,7 @@ def test_more_handed_over_rays_than_resume_records(gpu):
     handed, classified = C.coop_rays() - before, C.coop_interval_rays() - classified0
     s.close()
     print(f"rays handed over: {handed}, classified by intervals: {classified}")
-    assert handed > (64 << 20) // (80 + 12 * 10)
+    assert handed > (64 << 20) // 8 // 16
     assert classified > 0.99 * handed
     _same(got, ref)
 
