@@ -13,8 +13,10 @@
 //   [..., + kNoAllocResumeBytes)          resume area (16 MiB)
 // The list has room for every ray of a call of up to kNoAllocMaxRays rays,
 // and larger calls are rejected.  The resume area holds the state of the
-// first rays handed over (about 80,000 at 13 operator levels); later ones
-// restart from the root in the cooperative kernel, with the same results.
+// first rays handed over (about 73,000 with 10 stack slots) and, in its last
+// eighth, the directions of the first 131,072; later ones restart from the
+// root, or are built again from their pixel, in the cooperative kernel, with
+// the same results.
 //
 // All calls on one GPU share its array: calls that overlap on one GPU (two
 // streams, two host threads) race.  The kernels return the queue words to
