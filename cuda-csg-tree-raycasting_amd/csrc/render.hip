@@ -1429,9 +1429,10 @@ __device__ Hit CoopTrace(const SceneView& sv, const RayF& r, OpState* st, uint16
 //      flip parity changes inside the group, so the group's mask is the or
 //      of its members' masks: one walk iteration for the group where the
 //      tree takes one a level;
-//   B. the wave sorts the k <= 31 crossings by time (one per lane, each placed by a count);
-//      interval i of the ray is the stretch before crossing i (i = k: after
-//      the last), each crossed primitive's membership over the intervals
+//   B. the wave ranks the k <= 31 crossings by time (one per lane, each
+//      ranked by a count, ties sent to the state machine); interval i of the
+//      ray is the stretch before the crossing of rank i (i = k: after the
+//      last), each crossed primitive's membership over the intervals
 //      becomes a 32-bit mask of its own (bits after its Enter, up to its
 //      Exit), and the masks are or'ed into their owners' child slots;
 //   C. the operators combine their children's masks bottom-up (Union: or,
@@ -1537,7 +1538,7 @@ __device__ int IntervalTrace(const SceneView& sv, const RayF& r, void* region, i
   uint32_t nCross = 0, nEntered = 0, pend = 0, nPrims = 0;  // wave-uniform
   // Tests the pending primitive tests from the first, 64 a batch, while more
   // than `keep` are pending, appends their crossings in the stretch (in any
-  // order: B sorts them), numbering each crossed primitive for its mask, and
+  // order: B ranks them), numbering each crossed primitive for its mask, and
   // moves the untested rest (fewer than 64) to the front.  Returns 0, or the
   // outcome that ends the walk (kIvFull, kIvFail).
   auto drain = [&](uint32_t keep) -> int {
