@@ -206,7 +206,7 @@ constexpr size_t kCoopJobs = 192;       // pending primitive tests: fewer than 6
 // of a group's record, the members one chunk box covers, and the hit chunks
 // of one round of chunk tests (kept in LDS, 16 bits each).
 constexpr uint32_t kWalkGroupFlag = 0x200u;
-constexpr uint32_t kWalkChunk = 8;
+constexpr uint32_t kWalkChunk = 4;
 constexpr size_t kCoopHitChunks = 64;
 // Rays the cooperative kernel has finished wait after the region (heavy slot,
 // hit time, hit bits; 16 B each) until kCoopShadeBatch of them are shaded at

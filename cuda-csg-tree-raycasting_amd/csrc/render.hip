@@ -1423,7 +1423,7 @@ __device__ Hit CoopTrace(const SceneView& sv, const RayF& r, OpState* st, uint16
 //      state machine's.  A Union group (a Union subtree flattened by the host
 //      into its members, the nodes under its Unions that are not Unions) is
 //      one ENTER: the whole wave tests the boxes of its members' chunks (the
-//      host orders members spatially and boxes each 8), then the boxes of
+//      host orders members spatially and boxes each 4), then the boxes of
 //      the members of the hit chunks, 64 a round.  A Union's solid is the
 //      union of its members' however the Unions pair them, and no member's
 //      flip parity changes inside the group, so the group's mask is the or
@@ -1659,9 +1659,10 @@ __device__ int IntervalTrace(const SceneView& sv, const RayF& r, void* region, i
     WaveSync();
     // the Union groups entered: the boxes of their chunks (kWalkChunk members
     // each), 64 a round, the hit ones listed in LDS; then the members of the
-    // hit chunks, 64 (8 chunks) a round, two rounds' records loading at once
-    // (unconditional loads: lanes without a member read the group's last
-    // one and ignore it, so the compiler counts the loads exactly).  Hit
+    // hit chunks, 64 (64 / kWalkChunk chunks) a round, two rounds' records
+    // loading at once (unconditional loads: lanes without a member read the
+    // group's last one and ignore it, so the compiler counts the loads
+    // exactly).  Hit
     // primitives join the pending tests as the group's left child, hit
     // operators are queued (at most 64 tests are pending before each pair
     // of rounds: room for both).
@@ -1706,14 +1707,17 @@ __device__ int IntervalTrace(const SceneView& sv, const RayF& r, void* region, i
         if (hit) hc[LanesBelow(hm)] = static_cast<uint16_t>(c + lane);
         const uint32_t nHit = static_cast<uint32_t>(__popcll(hm));
         WaveSync();
-        // lane l takes member l % 8 of hit chunk h + l / 8 (count: none)
+        // lane l takes member l % kWalkChunk of hit chunk h + l / kWalkChunk
+        // (count: none)
+        constexpr uint32_t kPerRound = 64 / kWalkChunk;  // hit chunks a round
+        static_assert(kWalkChunk <= 64 && (kWalkChunk & (kWalkChunk - 1)) == 0, "chunks tile the wave");
         auto member = [&](uint32_t h) -> uint32_t {
-          const uint32_t k = h + (static_cast<uint32_t>(lane) >> 3);
-          const uint32_t mi = hc[min(k, nHit - 1)] * kWalkChunk + (static_cast<uint32_t>(lane) & 7u);
+          const uint32_t k = h + static_cast<uint32_t>(lane) / kWalkChunk;
+          const uint32_t mi = hc[min(k, nHit - 1)] * kWalkChunk + static_cast<uint32_t>(lane) % kWalkChunk;
           return k < nHit && mi < count ? mi : count;
         };
-        for (uint32_t h = 0; h < nHit; h += 16) {
-          const uint32_t ia = member(h), ib = member(h + 8);
+        for (uint32_t h = 0; h < nHit; h += 2 * kPerRound) {
+          const uint32_t ia = member(h), ib = member(h + kPerRound);
           float4 a0, a1, b0, b1;
           load(mfirst + 2 * static_cast<size_t>(min(ia, count - 1)), a0, a1);
           load(mfirst + 2 * static_cast<size_t>(min(ib, count - 1)), b0, b1);
@@ -1722,7 +1726,7 @@ __device__ int IntervalTrace(const SceneView& sv, const RayF& r, void* region, i
             if (const int e = drain(63)) return e;
           }
           round(ia < count, a0, a1);
-          if (h + 8 >= nHit) break;
+          if (h + kPerRound >= nHit) break;
           round(ib < count, b0, b1);
         }
         WaveSync();

@@ -118,7 +118,7 @@ struct FlatScene {
 
 constexpr uint32_t kGroupFlag = 0x200u;    // type word bit of a flattened Union group (walkOps)
 constexpr int32_t kMaxGroupMembers = 1024;  // a group's members are tested 64 a round by one wave
-constexpr int32_t kGroupChunk = 8;          // members under one chunk box
+constexpr int32_t kGroupChunk = 4;          // members under one chunk box
 
 constexpr int32_t kMaxNodes = 65535;       // 16-bit node links
 constexpr int32_t kMaxPrimitives = 32767;  // the reference stores hit ids as int16 (Ray.cuh:41)
