@@ -218,6 +218,8 @@ using namespace csgray::rt;
 
 static_assert(csgray::kGroupFlag == csgray::kWalkGroupFlag && csgray::kGroupChunk == csgray::kWalkChunk,
               "the walk records' layout is shared by the scene build and the kernels");
+static_assert(csgray::kMaxGroupMembers <= static_cast<int32_t>(csgray::kWalkMaxChunks * csgray::kWalkChunk),
+              "a group's chunk numbers fit the kernels' one-byte hit-chunk list");
 
 extern "C" {
 

@@ -191,7 +191,7 @@ size_t CoopLdsBytes(int32_t ops, int32_t leaves, bool lists, uint32_t* queueCap)
 // 16-byte state per operator, then the task ring's float tmins) and, when
 // `lists`, interval classification's (per operator two 4-byte child masks,
 // a 4-byte word and a 2-byte entry slot; then the crossings, 12 bytes each,
-// the pending primitive tests, 8 bytes each, and a round's hit chunks);
+// the pending primitive tests, 8 bytes each, and two rounds' hit chunks);
 // the ring's 16-bit tasks end the region in both.
 #if defined(__HIPCC__)
 #define CSG_HOST_DEVICE __host__ __device__
@@ -204,10 +204,12 @@ constexpr uint32_t kCoopMaxCross = 31;  // crossings a ray's interval masks (32 
 constexpr size_t kCoopJobs = 192;       // pending primitive tests: fewer than 64 left over + two a lane
 // Union groups in the interval walk's records (scene.hpp): the type-word flag
 // of a group's record, the members one chunk box covers, and the hit chunks
-// of one round of chunk tests (kept in LDS, 16 bits each).
+// of two rounds of chunk tests (kept in LDS, one byte each: a group has at
+// most kWalkMaxChunks chunks).
 constexpr uint32_t kWalkGroupFlag = 0x200u;
 constexpr uint32_t kWalkChunk = 4;
-constexpr size_t kCoopHitChunks = 64;
+constexpr uint32_t kWalkMaxChunks = 256;
+constexpr size_t kCoopHitChunks = 128;
 // Rays the cooperative kernel has finished wait after the region (heavy slot,
 // hit time, hit bits; 16 B each) until kCoopShadeBatch of them are shaded at
 // once, one a lane; then 32 B of state for a ray classified in stretches.
@@ -222,7 +224,7 @@ CSG_HOST_DEVICE inline size_t CoopWaveLds(int32_t ops, uint32_t queueCap, int32_
   const size_t machine =
       static_cast<size_t>(ops) * kCoopOpStateBytes + static_cast<size_t>(queueCap) * kCoopTaskBytes;
   const size_t masks = CoopCrossOffset(ops) + 12 * (static_cast<size_t>(kCoopMaxCross) + 1) + 8 * kCoopJobs +
-                       2 * kCoopHitChunks + static_cast<size_t>(queueCap) * 2;
+                       kCoopHitChunks + static_cast<size_t>(queueCap) * 2;
   const size_t bytes = lists && masks > machine ? masks : machine;
   return (bytes + 15) & ~static_cast<size_t>(15);
 }

@@ -1464,8 +1464,8 @@ __device__ Hit CoopTrace(const SceneView& sv, const RayF& r, OpState* st, uint16
 // entered it (18-31); then the entered operators in order (16 bits each), the
 // crossings (time, hit bits, tag: owner = operator << 2 | side << 1 | flip
 // parity, the crossed primitive's number << 24, 1 << 29 for an Exit alone),
-// the pending primitive tests (owner, descriptor), one round's hit chunks of
-// a Union group (16 bits each), and the task ring (ENTER tasks: operator |
+// the pending primitive tests (owner, descriptor), two rounds' hit chunks of
+// a Union group (8 bits each), and the task ring (ENTER tasks: operator |
 // flip parity << 15) at the region's end.  A crossed primitive's number in
 // its crossings' tags tells its Exit (appended right after its Enter) from
 // a neighbour's crossing with the same owner.
@@ -1534,7 +1534,7 @@ __device__ int IntervalTrace(const SceneView& sv, const RayF& r, void* region, i
   uint32_t* cb = reinterpret_cast<uint32_t*>(ct + kCoopMaxCross + 1);
   uint32_t* co = cb + kCoopMaxCross + 1;
   uint2* jobs = reinterpret_cast<uint2*>(co + kCoopMaxCross + 1);  // kCoopJobs pending primitive tests
-  uint16_t* hc = reinterpret_cast<uint16_t*>(jobs + kCoopJobs);     // kCoopHitChunks hit chunks
+  uint8_t* hc = reinterpret_cast<uint8_t*>(jobs + kCoopJobs);       // kCoopHitChunks hit chunks
   uint32_t nCross = 0, nEntered = 0, pend = 0, nPrims = 0;  // wave-uniform
   // Tests the pending primitive tests from the first, 64 a batch, while more
   // than `keep` are pending, appends their crossings in the stretch (in any
@@ -1658,7 +1658,7 @@ __device__ int IntervalTrace(const SceneView& sv, const RayF& r, void* region, i
     pend += nl + static_cast<uint32_t>(__popcll(bR));
     WaveSync();
     // the Union groups entered: the boxes of their chunks (kWalkChunk members
-    // each), 64 a round, the hit ones listed in LDS; then the members of the
+    // each), 128 at a time, the hit ones listed in LDS; then the members of the
     // hit chunks, 64 (64 / kWalkChunk chunks) a round, two rounds' records
     // loading at once (unconditional loads: lanes without a member read the
     // group's last one and ignore it, so the compiler counts the loads
@@ -1698,14 +1698,21 @@ __device__ int IntervalTrace(const SceneView& sv, const RayF& r, void* region, i
         if (job != kNone) jobs[pend + LanesBelow(bj)] = make_uint2(job, d);
         pend += static_cast<uint32_t>(__popcll(bj));
       };
-      for (uint32_t c = 0; c < nChunks; c += 64) {
-        float4 c0, c1;
-        load(first + 2 * static_cast<size_t>(min(c + static_cast<uint32_t>(lane), nChunks - 1)), c0, c1);
-        const bool hit =
-            c + lane < nChunks && BoxHit<true>(r, c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, 0.0f);
-        const uint64_t hm = __ballot(hit);
-        if (hit) hc[LanesBelow(hm)] = static_cast<uint16_t>(c + lane);
-        const uint32_t nHit = static_cast<uint32_t>(__popcll(hm));
+      // two rounds of chunk boxes load at once (lanes past the last chunk read
+      // it and ignore it), and their hit chunks are listed together
+      for (uint32_t c = 0; c < nChunks; c += 128) {
+        const uint32_t ca = c + static_cast<uint32_t>(lane), cb = ca + 64;
+        float4 c0, c1, d0, d1;
+        load(first + 2 * static_cast<size_t>(min(ca, nChunks - 1)), c0, c1);
+        load(first + 2 * static_cast<size_t>(min(cb, nChunks - 1)), d0, d1);
+        const bool hitA = ca < nChunks && BoxHit<true>(r, c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, 0.0f);
+        const uint64_t ha = __ballot(hitA);
+        if (hitA) hc[LanesBelow(ha)] = static_cast<uint8_t>(ca);
+        const uint32_t na = static_cast<uint32_t>(__popcll(ha));
+        const bool hitB = cb < nChunks && BoxHit<true>(r, d0.x, d0.y, d0.z, d0.w, d1.x, d1.y, 0.0f);
+        const uint64_t hb = __ballot(hitB);
+        if (hitB) hc[na + LanesBelow(hb)] = static_cast<uint8_t>(cb);
+        const uint32_t nHit = na + static_cast<uint32_t>(__popcll(hb));
         WaveSync();
         // lane l takes member l % kWalkChunk of hit chunk h + l / kWalkChunk
         // (count: none)
