@@ -167,6 +167,23 @@ __device__ __forceinline__ float TruncInt(float q) {
   return fminf(fmaxf(truncf(q), -2147483648.0f), 2147483647.0f);
 }
 
+// TruncInt(p / half * 1.00001f), a component of the cube's face normal at a
+// point p from its centre (cubeHit :422-424, hit details :447-449): the
+// quotient taken with the hardware reciprocal of half is within 1e-6 of the
+// IEEE one (the reciprocal's 1 ulp and three roundings), so wherever it lies
+// more than 4e-6 from 1 and below 2 in magnitude it truncates the same way
+// (to 0 or 1 with p's sign, the sign of zero included); only there, or for
+// a NaN or an infinity, is the IEEE division taken.  `rh` is the hardware
+// reciprocal of `half`.
+__device__ __forceinline__ float CubeNormalAxis(float p, float half, float rh) {
+  constexpr float kBias = 1.00001f;
+  const float x = p * rh * kBias;
+  const float ax = fabsf(x);
+  if (ax < 1.0f - 4e-6f) return copysignf(0.0f, x);
+  if (ax > 1.0f + 4e-6f && ax < 2.0f - 8e-6f) return copysignf(1.0f, x);
+  return TruncInt(p / half * kBias);
+}
+
 // Slab limits of an axis-aligned box (isBVHNodeHit :722-732, cubeHit :385-397).
 template <bool kFast>
 __device__ __forceinline__ void Slab(const RayF& r, float lx, float ly, float lz, float hx, float hy, float hz,
@@ -392,12 +409,11 @@ __device__ __forceinline__ bool CubeSlab(const RayF& r, const float4& a, float& 
 
 // The cube's hit record at slab limit `t` (the face normal by truncation).
 __device__ __forceinline__ Hit CubeAt(const RayF& r, const float4& a, uint32_t prim, float t) {
-  const float half = a.w / 2;
+  const float half = a.w / 2, rh = __builtin_amdgcn_rcpf(half);
   const float px = (r.ox + t * r.dx) - a.x;
   const float py = (r.oy + t * r.dy) - a.y;
   const float pz = (r.oz + t * r.dz) - a.z;
-  const float bias = 1.00001f;
-  const float nx = TruncInt(px / half * bias), ny = TruncInt(py / half * bias), nz = TruncInt(pz / half * bias);
+  const float nx = CubeNormalAxis(px, half, rh), ny = CubeNormalAxis(py, half, rh), nz = CubeNormalAxis(pz, half, rh);
   const float dn = nx * r.dx + ny * r.dy + nz * r.dz;
   return Hit{t, HitBits(dn <= 0 ? kEnter : kExit, kCubeT, prim)};
 }
@@ -778,11 +794,10 @@ __device__ __forceinline__ void ShadeAndStore(const Job& jb, const SceneView& sv
         Normalize3<kFast>(Nx, Ny, Nz);
       }
     } else {
-      const float half = a.w / 2;
-      const float bias = 1.00001f;
-      Nx = TruncInt((px - a.x) / half * bias);
-      Ny = TruncInt((py - a.y) / half * bias);
-      Nz = TruncInt((pz - a.z) / half * bias);
+      const float half = a.w / 2, rh = __builtin_amdgcn_rcpf(half);
+      Nx = CubeNormalAxis(px - a.x, half, rh);
+      Ny = CubeNormalAxis(py - a.y, half, rh);
+      Nz = CubeNormalAxis(pz - a.z, half, rh);
       Normalize3<kFast>(Nx, Ny, Nz);
     }
     if (flags & kFlip) { Nx = -Nx; Ny = -Ny; Nz = -Nz; }

@@ -270,6 +270,35 @@ def test_heavy_list_overflow_finishes_per_lane(gpu):
     _same(c, b, ("rgba", "depth", "prim_id", "flags", "counters"))
 
 
+@pytest.mark.parametrize("target", [(0.99999, 0.0, 1.0), (0.99999, 0.99999, 1.0), (-0.99999, 0.99999, 1.0)],
+                         ids=["edge", "corner", "corner_negative"])
+def test_cube_normals_at_the_truncation_boundary(gpu, target):
+    """The cube's face normal truncates p / half * 1.00001 (half = 1 here),
+    which changes value at |p| = 0.99999: a camera with a 1e-4 degree field
+    of view looking straight down at such a point puts every ray's hit within
+    a few millionths of it, where the kernels take the IEEE division instead
+    of the hardware reciprocal.  Exact mode equals the FP32 oracle bit for
+    bit, every ray per lane and every ray handed over."""
+    print(f"\n[case] cube face at {target}, 1e-4 degree view, exact", flush=True)
+    text = "Union\nCube 0 0 0 FFFF00 2\nCube 4 4 4 00FFFF 0.1\n"
+    eye = (target[0], target[1], 5.0)
+    cam = C.camera_look_at(eye, target, 1e-4)
+    sname, _, light, phong, _, l3, ph8 = setups()[1]
+    s = C.Scene(text)
+    o = O.OracleScene(text).render(C.cam_tuple(cam), l3, ph8, 64, 64, bits=32, counters=True)
+    assert (o["prim_id"] == 0).all()
+    for coop, when in ((-1, 0), (1, 1)):
+        g = _np(s.render(cam, light, phong, 64, 64, mode=C.MODE_EXACT, counters=True, coop=coop, coop_when=when))
+        assert np.array_equal(g["prim_id"], o["prim_id"])
+        assert np.array_equal(g["flags"], o["flags"])
+        assert np.array_equal(g["depth"].view(np.uint32), o["depth"].view(np.uint32))
+        assert np.array_equal(g["counters"], o["counters"])
+        assert np.abs(g["rgba"] - o["rgba"]).max() <= 2e-6
+    fast = [_np(s.render(cam, light, phong, 64, 64, coop=coop, coop_when=when)) for coop, when in ((-1, 0), (1, 1))]
+    s.close()
+    _same(fast[0], fast[1])
+
+
 @pytest.mark.parametrize("name", ["testWikipedia.txt", "testCheese512.txt"])
 def test_deferred_jittered_samples_match_oracle(gpu, name):
     print(f"\n[case] {name} 4 spp budget 1", flush=True)
