@@ -1794,6 +1794,7 @@ __device__ int IntervalTrace(const SceneView& sv, const RayF& r, void* region, i
     tag = co[lane];
   }
   uint32_t place = 0;
+#pragma unroll 2
   for (uint32_t i = 0; i < k; i++)
     place += __uint_as_float(__builtin_amdgcn_readlane(AsU(key), static_cast<int>(i))) < key ? 1u : 0u;
   uint32_t* claim = reinterpret_cast<uint32_t*>(ct);  // place < k <= 31 for a held crossing
@@ -1806,7 +1807,13 @@ __device__ int IntervalTrace(const SceneView& sv, const RayF& r, void* region, i
   // right after the Enter, so it is the next lane's, with the same tag --,
   // or all set up to an Exit alone.
   const bool enter = (bits & kEnter) != 0;
-  const uint32_t nextTag = __shfl_down(tag, 1), nextPlace = __shfl_down(place, 1);
+  // (lane i reads lane i + 1's tag and place with a whole-wave DPP shift:
+  // a VALU move, where a shuffle takes an LDS round trip)
+  constexpr int kDppWaveShl1 = 0x130;
+  const uint32_t nextTag = static_cast<uint32_t>(
+      __builtin_amdgcn_update_dpp(0, static_cast<int>(tag), kDppWaveShl1, 0xF, 0xF, false));
+  const uint32_t nextPlace = static_cast<uint32_t>(
+      __builtin_amdgcn_update_dpp(0, static_cast<int>(place), kDppWaveShl1, 0xF, 0xF, false));
   if (held && (enter || (tag & kIvLone))) {
     uint32_t m = enter ? (~0u << (place + 1)) : ((2u << place) - 1u);
     if (enter && static_cast<uint32_t>(lane) + 1 < k && nextTag == tag) m &= (2u << nextPlace) - 1u;
