@@ -1927,8 +1927,8 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) Co
   // heavy-list entry read while the current ray is shaded (`entry`).
   uint32_t ahead = kNone;
   uint2 entry = make_uint2(0u, 0u);
-  // the direction the per-lane kernel left in the slot's resume record
-  // (slots below dirCap have one), read with the entry
+  // the direction the per-lane kernel left for the slot in the resume area's
+  // direction array (slots below dirCap have one), read with the entry
   const uint32_t dirCap = DirCap(p);
   uint4 dirw = make_uint4(0u, 0u, 0u, 0u);
   auto dirOf = [&](const RenderParams& pp, uint64_t slot) {
