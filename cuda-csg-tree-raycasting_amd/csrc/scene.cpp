@@ -411,26 +411,39 @@ FlatScene Flatten(const CSGTree& tree) {
         continue;
       }
       if (members.size() < 3) continue;
-      // Morton order of the members' box centres within the group's box
-      // (10 bits an axis), then a box around each chunk of kGroupChunk
-      const Box& gb = tight[g];
-      std::vector<std::pair<uint32_t, int32_t>> keyed;
-      keyed.reserve(members.size());
-      for (const int32_t m : members) {
-        uint32_t code = 0;
-        for (int a = 0; a < 3; a++) {
-          const double span = static_cast<double>(gb.hi[a]) - gb.lo[a];
-          const double mid = 0.5 * (static_cast<double>(tight[m].lo[a]) + tight[m].hi[a]);
-          double u = span > 0.0 ? (mid - gb.lo[a]) / span : 0.0;
-          u = std::isfinite(u) ? std::min(1.0, std::max(0.0, u)) : 0.0;
-          const uint32_t qv = std::min<uint32_t>(1023u, static_cast<uint32_t>(u * 1024.0));
-          for (int bit = 0; bit < 10; bit++) code |= ((qv >> bit) & 1u) << (3 * bit + a);
+      // Members ordered by recursive median splits of their box centres
+      // (along the centres' longest extent; the lower part a whole number
+      // of chunks), so that each chunk of kGroupChunk holds near neighbours:
+      // on the 512-sphere cheese a ray through the cube meets 9.1 of the 128
+      // chunk boxes on average, against 12.7 in Morton order.  Then a box
+      // around each chunk.
+      auto centre = [&](int32_t m, int a) {
+        const double v = 0.5 * (static_cast<double>(tight[m].lo[a]) + tight[m].hi[a]);
+        return std::isfinite(v) ? v : 0.0;  // a finite key keeps the sort's ordering strict
+      };
+      std::vector<std::pair<size_t, size_t>> spans{{0, members.size()}};
+      while (!spans.empty()) {
+        const auto [b, e] = spans.back();
+        spans.pop_back();
+        if (e - b <= static_cast<size_t>(kGroupChunk)) continue;
+        double lo[3], hi[3];
+        for (int a = 0; a < 3; a++) lo[a] = hi[a] = centre(members[b], a);
+        for (size_t i = b + 1; i < e; i++) {
+          for (int a = 0; a < 3; a++) {
+            lo[a] = std::min(lo[a], centre(members[i], a));
+            hi[a] = std::max(hi[a], centre(members[i], a));
+          }
         }
-        keyed.emplace_back(code, m);
+        int ax = 0;
+        for (int a = 1; a < 3; a++) {
+          if (hi[a] - lo[a] > hi[ax] - lo[ax]) ax = a;
+        }
+        std::stable_sort(members.begin() + static_cast<std::ptrdiff_t>(b), members.begin() + static_cast<std::ptrdiff_t>(e),
+                         [&](int32_t x, int32_t y) { return centre(x, ax) < centre(y, ax); });
+        const size_t lower = ((e - b) / 2 + kGroupChunk - 1) / kGroupChunk * kGroupChunk;
+        spans.push_back({b, b + lower});
+        spans.push_back({b + lower, e});
       }
-      std::stable_sort(keyed.begin(), keyed.end(),
-                       [](const auto& x, const auto& y) { return x.first < y.first; });
-      for (size_t i = 0; i < keyed.size(); i++) members[i] = keyed[i].second;
       const uint32_t gi = static_cast<uint32_t>(opIndex[g]);
       const size_t first = f.walkOps.size();
       for (size_t c0 = 0; c0 < members.size(); c0 += kGroupChunk) {

@@ -88,8 +88,8 @@ struct CSGTree {
 // members, (lo.x, lo.y, lo.z, hi.x), (hi.y, hi.z, 0, 0), the box around
 // those members, and then one record per member,
 //   (lo.x, lo.y, lo.z, hi.x), (hi.y, hi.z, bits(desc), 0),
-// with its culling box, the members in Morton order of their boxes' centres
-// (so that a chunk's members lie close together); a member that is an
+// with its culling box, the members ordered by recursive median splits of
+// their boxes' centres (so that a chunk's members lie close together); a member that is an
 // operator names the group as its parent (as a left child).  A larger Union
 // subtree stays binary at the top and its Union children are tried as
 // groups.
